@@ -1,0 +1,178 @@
+/*
+ * xt_icd.h -- C-ABI of libxt_hip.so, the MI355X (gfx950) ICD engine.
+ *
+ * Plain C: structs from xt_structures.h, raw pointers and sizes; no torch or
+ * HIP types.  Every entry point returns 0 on success and a non-zero XT_E*
+ * code on failure (the reference exits the process instead; a library must
+ * not), with the message retrievable from xt_last_error().
+ *
+ * Drop-in entry point
+ *   ICD_BackProject       replaces Source_Code_4D_Fast/XT_ICD_update.h:40
+ *                         (defined XT_ICD_update.c:928-1094)
+ *
+ * Engine around it (the reference's XT_Engine.c main, for hosts that drive
+ * the engine through its command line and files)
+ *   xt_engine_setup / xt_engine_read_sinogram / xt_engine_write_outputs /
+ *   xt_engine_free / xt_engine_run
+ *
+ * Test seams: one entry per hot-path function, each computing on the GPU what
+ * the cited reference function computes on the CPU, with host arrays in and
+ * out.  They allocate device memory per call and are not meant to be fast.
+ */
+#ifndef XT_ICD_H_AMD
+#define XT_ICD_H_AMD
+
+#include "xt_structures.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  XT_OK = 0,
+  XT_E_ARG = 1,      /* unsupported or inconsistent input */
+  XT_E_HIP = 2,      /* HIP runtime error (no device, out of memory, launch failure) */
+  XT_E_IO = 3,       /* file could not be read or written */
+};
+
+/* Message for the last non-zero return on this thread ("" if none). */
+const char *xt_last_error(void);
+
+/* Library build identification, e.g. "libxt_hip gfx950 Real_t=float". */
+const char *xt_version(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in boundary.  Same contract as the reference function:
+ *  - reads Sinogram {Projection, ProjOffset, cosine, sine, ViewPtr, N_p, N_r,
+ *    N_t, delta_r, delta_t, R0, RMax, OffsetR, z_overlap_num}, ScannedObject
+ *    {N_x, N_y, N_z, N_time, x0, y0, delta_xy, delta_z, mult_xy, BeamWidth,
+ *    MRF_P, C_S, C_T, ProjIdxPtr, ProjNum, NHICD_Iterations} and TomoInputs
+ *    {Weight, var_est, Sigma_*_Q*, Spatial_Filter, Time_Filter, alpha, NumIter,
+ *    StopThreshold, radius_obj, ErrorSinoThresh, ErrorSinoDelta,
+ *    updateProjOffset, EnforceZeroMeanOffset, updateVar, no_NHICD, initICD,
+ *    num_z_blocks, node_rank, node_num};
+ *  - writes Object slices 1..N_z of every frame, ProjOffset, ProjSelect,
+ *    Weight and var_est (when the variance is re-estimated), and the files
+ *    variance_estimate_n<rank>.bin and mag_update_map_n<rank>.bin in the
+ *    current directory;
+ *  - frees SinogramPtr->Projection (pointer tree, F/XT_ICD_update.c:999);
+ *  - returns 0 on success.
+ * One process, one GPU (device 0 unless xt_set_option(XT_OPT_DEVICE) says
+ * otherwise).  Not reentrant. */
+int ICD_BackProject(Sinogram *SinogramPtr, ScannedObject *ScannedObjectPtr,
+                    TomoInputs *TomoInputsPtr);
+
+/* Options the reference fixes at compile time (-D macros) or not at all. */
+enum {
+  XT_OPT_POSITIVITY = 1,     /* POSITIVITY_CONSTRAINT: clamp voxels at 0 (default 0) */
+  XT_OPT_COST = 2,           /* compute the MAP cost each iteration, i.e. a build without
+                                NO_COST_CALCULATE (default 0) */
+  XT_OPT_RESYNC_EVERY = 3,   /* FP64 recompute of e = y - Ax - d every n iterations; 0 = never
+                                (default 0) */
+  XT_OPT_SEED = 4,           /* seed of the per-tile line order (default 761521) */
+  XT_OPT_TILE = 5,           /* super-voxel tile side in voxel columns; 0 = automatic */
+  XT_OPT_DEVICE = 6,         /* HIP device ordinal (default 0) */
+  XT_OPT_VERBOSE = 7,        /* per-iteration lines to the debug file (default 1) */
+  XT_OPT_OUT_DIR = 8         /* reserved */
+};
+int xt_set_option(int option, double value);
+double xt_get_option(int option);
+
+/* ------------------------------------------------------------------------
+ * Stepwise form of ICD_BackProject for hosts that time or steer iterations
+ * themselves (the benchmark).  xt_icd_create uploads the inputs, builds the
+ * detector table and the error sinogram; xt_icd_iterate runs n iterations
+ * (returns 1 in *stop when the stop threshold was met); xt_icd_download writes
+ * the results into the structs as ICD_BackProject does.  Projection is not
+ * freed by this form. */
+typedef struct xt_icd_state xt_icd_state;
+
+typedef struct {
+  int32_t iterations;           /* iterations run so far */
+  double voxel_updates;         /* performed (not zero-skipped) voxel updates */
+  double footprint_entries;     /* sum over performed updates of the sinogram
+                                   footprint F_v = sum_views count * z_overlap_num */
+  double icd_kernel_ms;         /* summed duration of the voxel-update launches
+                                   (valid when timing is on) */
+  int64_t icd_launches;
+  double last_cost;             /* MAP cost after the last iteration (XT_OPT_COST) */
+  double stop_percentage;       /* 100 * sum|dx| / sum|x|, the reference's stop test */
+  int32_t stopped;
+} xt_icd_stats;
+
+int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state);
+int xt_icd_iterate(xt_icd_state *state, int32_t n_iter, int32_t *stop);
+int xt_icd_set_timing(xt_icd_state *state, int32_t on);
+int xt_icd_get_stats(xt_icd_state *state, xt_icd_stats *stats);
+int xt_icd_cost(xt_icd_state *state, double *cost);          /* FP64 MAP cost now */
+int xt_icd_resync(xt_icd_state *state);                       /* FP64 e = y - Ax - d now */
+int xt_icd_download(xt_icd_state *state, Sinogram *S, ScannedObject *O, TomoInputs *T);
+int xt_icd_synchronize(xt_icd_state *state);
+void xt_icd_destroy(xt_icd_state *state);
+
+/* ------------------------------------------------------------------------
+ * Engine around the boundary: the reference's command line
+ * (F/XT_Init.c:452-629), initStructures (F/XT_Init.c:277-448), the binary
+ * inputs (F/XT_genSinogram.c:264-300, F/XT_Engine.c:120-123) and outputs
+ * (F/XT_IOMisc.c:224-254).  dir is the run folder (NULL or "" = cwd). */
+int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram *S, ScannedObject *O,
+                    TomoInputs *T);
+int xt_engine_read_sinogram(const char *dir, Sinogram *S, TomoInputs *T);
+int xt_engine_write_outputs(const char *dir, Sinogram *S, ScannedObject *O, TomoInputs *T);
+void xt_engine_free(Sinogram *S, ScannedObject *O, TomoInputs *T);
+/* setup + read + ICD_BackProject + write + free, as XT_Engine.c main. */
+int xt_engine_run(const char *dir, int argc, char **argv);
+
+/* Pointer-tree allocator with the reference's layout (F/allocate.c:91-188), so
+ * arrays the boundary frees (Projection) can be allocated by any caller. */
+void *xt_multialloc(size_t s, int d, const int64_t *dims);
+void xt_multifree(void *r, int d);
+
+/* ------------------------------------------------------------------------
+ * Test seams. */
+
+/* DetectorResponseProfile, F/XT_Profile.c:128-217: H_r[N_p][65]. */
+int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r);
+
+/* calcAMatrixColumnforAngle, F/XT_AMatrix.c:50-110, for n (row, col, view)
+ * triples at once: count[n], index[n][max_entries], values[n][max_entries]. */
+int xt_amatrix_columns(Sinogram *S, ScannedObject *O, Real_t **H_r, int32_t n,
+                       const int32_t *rows, const int32_t *cols, const int32_t *views,
+                       int32_t max_entries, uint8_t *count, int32_t *index, Real_t *values);
+
+/* Ax of every frame (initErrorSinogam's projection, F/XT_ICD_update.c:551-569,
+ * F/XT_ForwardProject.c:8-25), accumulated in FP64: AX64[N_p][N_r][N_t]. */
+int xt_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, double *AX64);
+
+/* initErrorSinogam, F/XT_ICD_update.c:538-597: E = y - Ax - d, ProjSelect. */
+int xt_init_error_sino(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r,
+                       Real_t ***E);
+
+/* computeCost, F/XT_ICD_update.c:181-314, in FP64. */
+int xt_compute_cost(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, double *cost);
+
+/* update_Sinogram_Offset, F/XT_ICD_update.c:666-759. */
+int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E);
+
+/* update_variance_parameter, F/XT_ICD_update.c:633-664. */
+int xt_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E);
+
+/* VSC_based_Voxel_Line_Select, F/XT_NHICD.c:164-216: fills NHICDSelectNum and
+ * x/y_NHICD_select from MagUpdateMap[N_time][num_z_blocks][N_y][N_x].  The
+ * threshold is the same order statistic; no random numbers are drawn. */
+int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap);
+
+/* updateVoxels + compute_voxel_update, F/XT_VoxUpdate.c:11-254: updates the
+ * listed voxel lines of one frame, one line at a time in list order, every
+ * slice of a line (slices in red-black order), with E kept in FP32.
+ * MagUpdate[N_y][N_x] receives the summed |change| of each updated line.
+ * Returns the summed |new value| of performed updates in *total_vox_mag. */
+int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, int32_t frame,
+                    int32_t n_lines, const int32_t *x_list, const int32_t *y_list, int32_t Iter,
+                    Real_t *MagUpdate, double *total_vox_mag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XT_ICD_H_AMD */

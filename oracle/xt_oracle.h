@@ -1,0 +1,109 @@
+/*
+ * xt_oracle.h -- CPU oracle for the TIMBIR ICD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (Source_Code_4D_Fast), used as the checker for the HIP library and
+ * as the timed CPU baseline.  Only tests/, the smoke check and the benchmark's
+ * CPU-baseline leg may load it; the product library never links or calls it.
+ *
+ * Built twice from one source: liboracle_f32.so (Real_t = float, the
+ * reference's production build) and liboracle_f64.so (XT_REAL_IS_DOUBLE, the
+ * reference's -DREAL_IS_DOUBLE build).
+ *
+ * Parity status of each function (how it is pinned):
+ *   oracle_random2/srandom2      known-answer table Source_Code_4D_Fast/randlib.c:69-91
+ *                                and bit-exact vs the reference's randlib.c (oracle/_ref)
+ *   oracle_detector_response     bit-exact vs the reference's XT_Profile.c (oracle/_ref)
+ *                                plus closed-form checks (symmetry, area)
+ *   oracle_amatrix_column        bit-exact vs the reference's XT_AMatrix.c (oracle/_ref)
+ *   oracle_forward_project       vs the reference's forward_project_voxel (oracle/_ref)
+ *                                summed over all voxels, and dense-matrix brute force
+ *   oracle_vsc_select            bit-exact vs the reference's XT_NHICD.c (oracle/_ref)
+ *   oracle_parse_view_info       the reference's view_info.txt fixtures
+ *   oracle_interlaced_views      golden log Source_Code_4D_Fast/info.txt:1-175
+ *   voxel update / surrogate     closed forms: quadratic prior + no zingers reduces ICD to
+ *                                Gauss-Seidel on (A'WA+H)x = A'W(y-d), solved densely;
+ *                                qGGMRF derivative vs finite differences of the value
+ *   offset / variance / cost     closed forms on small inputs (weighted means, sums)
+ *   full ICD driver              converges to the dense MAP solution (quadratic case);
+ *                                the reference's own ICD driver (XT_ICD_update.c) cannot
+ *                                be built here (it needs mpi.h), so end-to-end trajectory
+ *                                parity with the reference binary is unpinned.
+ */
+#ifndef XT_ORACLE_H
+#define XT_ORACLE_H
+
+#include "../include/xt_structures.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void oracle_srandom2(uint32_t seed);
+double oracle_random2(void);
+void oracle_random2_seq(uint32_t seed, int32_t n, double *out);
+
+void *oracle_multialloc(size_t s, int32_t d, const int64_t *dims);
+void oracle_multifree(void *r, int32_t d);
+
+void oracle_detector_response(Real_t **H_r, Sinogram *S, ScannedObject *O, TomoInputs *T);
+void oracle_amatrix_column(Sinogram *S, ScannedObject *O, Real_t **H_r, AMatrixCol *Ai,
+                           int32_t row, int32_t col, int32_t proj_idx);
+/* Ax for one frame's views, accumulated in Real_t into AX (zeroed first). */
+void oracle_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, Real_t ***AX);
+/* e = y - Ax - d and ProjSelect, as initErrorSinogam. */
+void oracle_init_error_sino(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r,
+                            Real_t ***E);
+Real_t oracle_compute_cost(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E);
+void oracle_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E);
+void oracle_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E);
+Real_t oracle_functional_substitution(Real_t V, Real_t THETA1, Real_t THETA2, ScannedObject *O,
+                                      TomoInputs *T, const Real_t *nhood27, const uint8_t *flag27,
+                                      const Real_t *time_nhood2, const uint8_t *time_flag2,
+                                      int32_t positivity);
+/* Updates the listed (y,x) voxel lines of frame i, slices [z0,z1], in list order. */
+Real_t oracle_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, Real_t **H_r,
+                           int32_t frame, int32_t z0, int32_t z1, int32_t n_lines,
+                           const int32_t *x_list, const int32_t *y_list, int32_t Iter,
+                           int64_t *zero_count, Real_t **MagUpdateMap, int32_t positivity);
+void oracle_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap);
+void oracle_random_select(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
+
+/* Full driver: same semantics as ICD_BackProject.  flags: bit0 compute the MAP
+ * cost every iteration (the reference's build without NO_COST_CALCULATE),
+ * bit1 positivity constraint.  cost_out (may be NULL) receives NumIter+1
+ * costs (index 0 = before ICD; unused entries NaN).  Returns iterations run. */
+int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, const char *dir,
+                               int32_t flags, double *cost_out);
+
+/* Whole engine: parse the reference CLI, read the inputs from dir, run ICD, write
+ * object_n0_time_<i>.bin and proj_offset_n0.bin into dir.  Returns 0 on success. */
+int32_t oracle_engine_run(const char *dir, int32_t argc, char **argv, int32_t flags,
+                          double *cost_out, int32_t *iters_out);
+
+/* Engine pieces: parse the reference CLI and run initStructures (reads
+ * view_info.txt and, when asked, proj_offset_n0.bin / variance_estimate_n0.bin
+ * from dir), read projection_n0.bin / weight_n0.bin, free what setup allocated. */
+int32_t oracle_engine_setup(const char *dir, int32_t argc, char **argv, Sinogram *S,
+                            ScannedObject *O, TomoInputs *T);
+int32_t oracle_engine_read_sinogram(const char *dir, Sinogram *S, TomoInputs *T);
+void oracle_engine_free(Sinogram *S, ScannedObject *O, TomoInputs *T);
+
+/* view_info.txt parse (returns views read, or -1 on a format error). */
+int32_t oracle_parse_view_info(const char *path, int32_t max_views, Real_t *views_rad,
+                               Real_t *times);
+/* Frame binning; proj_idx_flat is [N_time][N_p] row-major, proj_num [N_time]. */
+void oracle_bin_views(const Real_t *times, int32_t N_p, int32_t N_time, Real_t Rtime0,
+                      Real_t delta_Rtime, int32_t *proj_idx_flat, int32_t *proj_num);
+/* Interlaced bit-reversed order and clipping (4D_Recon_Launcher_for_Real_Data/
+ * XT_Interlaced_Angles.py:6-27,30-34,107-139).  Returns the kept count; the
+ * deleted views/times go to del_views/del_times, their count to *n_del. */
+int32_t oracle_interlaced_views(int32_t K, int32_t N_theta, int32_t N_p, double min_time,
+                                double rotation_speed, double *views, double *times,
+                                double *del_views, double *del_times, int32_t *n_del);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,283 @@
+"""GPU parity tests: every hot-path seam of libxt_hip.so against the CPU oracle
+on the same seeded inputs (and against the reference's own buildable sources
+in oracle/_ref where they cover the function).
+
+Tolerances (written per test):
+  bit-exact        detector table, A-matrix columns (bins and values), NHICD line sets
+  1e-5 relative    forward projection Ax (max |GPU - Ax_ref| / max |Ax|), Ax_ref = the
+                   float build's system matrix summed in FP64 (oracle f64a build)
+  1e-5 relative    FP64 MAP cost, ring offsets, variance
+  1e-4 relative    per-voxel updates (FP32 theta sums in a different order)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import SMALL, SMALL4D, TINY, make_scan, oracle_path
+from timbir_python_amd import xt_abi, xt_hip
+import py_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, cfg):
+    print(f"[gpu] {name}: {cfg.name} {cfg.N_xy}x{cfg.N_xy}x{cfg.N_z} x{cfg.N_time} views {cfg.N_p}",
+          flush=True)
+
+
+def _pair(cfg, scan_root, oracle):
+    path, info = make_scan(cfg, scan_root)
+    args = cfg.cli_args()
+    eng = xt_hip.Engine(path, args)
+    opath = oracle_path(path, oracle)
+    S, O, T = oracle.setup(opath, args)
+    oracle.read_sinogram(opath, S, T)
+    return path, info, eng, (S, O, T)
+
+
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D], ids=lambda c: c.name)
+def test_detector_response_bitexact(cfg, scan_root, oracle32):
+    _print_case("detector_response", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        Hg = xt_hip.detector_response(eng)
+        Ho = oracle32.detector_response(S, O, T)
+        assert np.array_equal(Hg.view(np.uint32), Ho.view(np.uint32))
+        if py_oracle.RefLib.available():
+            ref = py_oracle.RefLib(False)
+            Hr = np.zeros_like(Ho)
+            ht = xt_abi.PtrTree(Hr)
+            Ht = np.zeros(65, np.float32)
+            ref.lib.DetectorResponseProfile(ht.ptr, C.c_void_p(Ht.ctypes.data), C.byref(S),
+                                            C.byref(O), C.byref(T))
+            assert np.array_equal(Hg.view(np.uint32), Hr.view(np.uint32))
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+def test_amatrix_columns_bitexact(scan_root, oracle32):
+    cfg = TINY
+    _print_case("amatrix_columns", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        H = oracle32.detector_response(S, O, T)
+        rng = np.random.default_rng(0)
+        n = 4000
+        rows = rng.integers(0, O.N_y, n)
+        cols = rng.integers(0, O.N_x, n)
+        views = rng.integers(0, S.N_p, n)
+        # include the image corners and every view
+        rows[:S.N_p] = 0
+        cols[:S.N_p] = O.N_x - 1
+        views[:S.N_p] = np.arange(S.N_p)
+        cnt, idx, val = xt_hip.amatrix_columns(eng, H, rows, cols, views)
+        AM = oracle32.AMatrixCol
+        vals = np.zeros(8, np.float32)
+        inds = np.zeros(8, np.int32)
+        a = AM()
+        a.values = vals.ctypes.data
+        a.index = inds.ctypes.data
+        ht = xt_abi.PtrTree(H)
+        for i in range(n):
+            oracle32.lib.oracle_amatrix_column(C.byref(S), C.byref(O), ht.ptr, C.byref(a),
+                                               int(rows[i]), int(cols[i]), int(views[i]))
+            assert cnt[i] == a.count
+            k = a.count
+            assert np.array_equal(idx[i, :k], inds[:k])
+            assert np.array_equal(val[i, :k].view(np.uint32), vals[:k].view(np.uint32))
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D], ids=lambda c: c.name)
+def test_forward_project(cfg, scan_root, oracle32, oracle64a):
+    """Ax within 1e-5 relative of the float reference system matrix (FP64 sums).
+    The reference's double build uses a different detector table (its float
+    and double profiles differ by ~3e-3), so the float build is the target."""
+    _print_case("forward_project", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        vol = info["phantom"].astype(np.float32)
+        oracle32.set_object(O, vol)
+        eng.set_volume(vol)
+        H = oracle32.detector_response(S, O, T)
+        ax32 = oracle32.forward_project(S, O, H).astype(np.float64)
+        S2, O2, T2 = oracle64a.setup(oracle_path(path, oracle64a), cfg.cli_args())
+        try:
+            oracle64a.set_object(O2, vol.astype(np.float64))
+            ax_ref = oracle64a.forward_project(S2, O2, oracle64a.detector_response(S2, O2, T2))
+        finally:
+            oracle64a.free(S2, O2, T2)
+        axg = xt_hip.forward_project(eng, xt_hip.detector_response(eng))
+        rel = np.abs(axg - ax_ref).max() / np.abs(ax_ref).max()
+        rel32 = np.abs(ax32 - ax_ref).max() / np.abs(ax_ref).max()
+        print(f"  max|Ax_gpu - Ax_ref| / max|Ax| = {rel:.3e} (float-summed oracle: {rel32:.3e})")
+        assert rel <= 1e-5
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+def test_init_error_sino(scan_root, oracle32):
+    cfg = TINY
+    _print_case("init_error_sino", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        vol = (0.5 * info["phantom"]).astype(np.float32)
+        oracle32.set_object(O, vol)
+        eng.set_volume(vol)
+        H = oracle32.detector_response(S, O, T)
+        E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
+        et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+        oracle32.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+        sel_o = oracle32.sino_view(S.ProjSelect, S, bool).copy()
+        Eg = xt_hip.init_error_sino(eng, H)
+        sel_g = eng.proj_select().astype(bool)
+        scale = np.abs(eng.sino("Projection")).max()
+        assert np.abs(Eg - E).max() <= 1e-5 * scale
+        assert np.mean(sel_g != sel_o) <= 1e-4
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+def _error_sino_f64(oracle64, S, O, T, vol):
+    oracle64.set_object(O, vol)
+    H = oracle64.detector_response(S, O, T)
+    E = np.zeros((S.N_p, S.N_r, S.N_t), np.float64)
+    et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+    oracle64.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+    return E, et
+
+
+def test_cost(scan_root, oracle64a):
+    cfg = SMALL4D
+    _print_case("compute_cost", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle64a)
+    try:
+        vol = (0.8 * info["phantom"]).astype(np.float32)
+        E, et = _error_sino_f64(oracle64a, S, O, T, vol.astype(np.float64))
+        c64 = oracle64a.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        eng.set_volume(vol)
+        cg = xt_hip.compute_cost(eng, E.astype(np.float32))
+        print(f"  cost gpu {cg:.10g} oracle_f64 {c64:.10g}")
+        assert abs(cg - c64) <= 1e-5 * abs(c64)
+    finally:
+        eng.close()
+        oracle64a.free(S, O, T)
+
+
+def test_offset_update(scan_root, oracle64a):
+    cfg = TINY
+    _print_case("offset_update", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle64a)
+    try:
+        vol = (0.9 * info["phantom"]).astype(np.float32)
+        E, et = _error_sino_f64(oracle64a, S, O, T, vol.astype(np.float64))
+        T.EnforceZeroMeanOffset = 1
+        eng.T.EnforceZeroMeanOffset = 1
+        Eg = E.astype(np.float32)
+        oracle64a.lib.oracle_offset_update(C.byref(S), C.byref(T), et.ptr)
+        xt_hip.offset_update(eng, Eg)
+        d_o = oracle64a.offset_view(S)
+        d_g = eng.offset()
+        assert np.abs(d_g - d_o).max() <= 1e-5 * max(1e-3, np.abs(d_o).max())
+        assert np.abs(Eg - E).max() <= 1e-5 * np.abs(E).max()
+    finally:
+        eng.close()
+        oracle64a.free(S, O, T)
+
+
+def test_variance_update(scan_root, oracle64a):
+    cfg = TINY
+    _print_case("variance_update", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle64a)
+    try:
+        vol = info["phantom"].astype(np.float32)
+        E, et = _error_sino_f64(oracle64a, S, O, T, vol.astype(np.float64))
+        Eg = E.astype(np.float32)
+        oracle64a.lib.oracle_variance_update(C.byref(S), C.byref(T), et.ptr)
+        xt_hip.variance_update(eng, Eg)
+        print(f"  var gpu {eng.T.var_est:.8g} oracle {T.var_est:.8g}")
+        assert abs(eng.T.var_est - T.var_est) <= 1e-5 * T.var_est
+        wo = oracle64a.sino_view(T.Weight, S)
+        assert np.abs(eng.sino("Weight") - wo).max() <= 1e-5 * wo.max()
+    finally:
+        eng.close()
+        oracle64a.free(S, O, T)
+
+
+def test_vsc_select_bitexact(scan_root, oracle32):
+    cfg = SMALL4D
+    _print_case("vsc_select", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        nb = T.num_z_blocks
+        rng = np.random.default_rng(7)
+        mag = rng.exponential(1e-4, (O.N_time, nb, O.N_y, O.N_x)).astype(np.float32)
+        mag[..., :3, :] = 0  # ties at zero
+        usn = np.full((O.N_time, nb), O.N_y * O.N_x, np.int32)
+        got = xt_hip.vsc_select(eng, mag, usn)
+        usn_o = np.ctypeslib.as_array((C.c_int32 * (O.N_time * nb)).from_address(
+            C.cast(T.UpdateSelectNum, C.POINTER(C.c_void_p))[0]))
+        usn_o[:] = usn.ravel()
+        mt = xt_abi.PtrTree(mag.copy())
+        oracle32.lib.oracle_srandom2(761521)
+        oracle32.lib.oracle_vsc_select(C.byref(O), C.byref(T), mt.ptr)
+        nxy = O.N_y * O.N_x
+        num = py_oracle.deref_data(T.NHICDSelectNum, 1, C.c_int32, O.N_time * nb).reshape(O.N_time, nb)
+        xs = py_oracle.deref_data(T.x_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
+        ys = py_oracle.deref_data(T.y_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
+        for i in range(O.N_time):
+            for b in range(nb):
+                k = num[i, b]
+                gy, gx = got[i][b]
+                assert len(gy) == k
+                assert np.array_equal(gy, ys[i, b, :k]) and np.array_equal(gx, xs[i, b, :k])
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+@pytest.mark.parametrize("Iter", [1, 2])
+def test_update_lines_per_voxel(Iter, scan_root, oracle32):
+    cfg = TINY
+    _print_case(f"update_lines Iter={Iter}", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        vol = (0.7 * info["phantom"]).astype(np.float32)
+        vol[:, :, :4, :] = 0  # zero rows: zero-skipping paths
+        oracle32.set_object(O, vol)
+        eng.set_volume(vol)
+        H = oracle32.detector_response(S, O, T)
+        E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
+        et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+        oracle32.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+        Eg = E.copy()
+        rng = np.random.default_rng(Iter)
+        n = 40
+        ys = rng.integers(0, O.N_y, n).astype(np.int32)
+        xs = rng.integers(0, O.N_x, n).astype(np.int32)
+        ys[:4] = 1  # lines in the zero rows
+        mag_o = np.zeros((O.N_y, O.N_x), np.float32)
+        mt = xt_abi.PtrTree(mag_o)
+        zc = C.c_int64(0)
+        tot_o = oracle32.lib.oracle_update_lines(C.byref(S), C.byref(O), C.byref(T), et.ptr, ht.ptr,
+                                                 0, 0, O.N_z - 1, n, xs.ctypes.data, ys.ctypes.data,
+                                                 Iter, C.byref(zc), mt.ptr, 0)
+        mag_g, tot_g = xt_hip.update_lines(eng, Eg, 0, xs, ys, Iter)
+        xo = oracle32.get_object(O)
+        xg = eng.volume()
+        scale = np.abs(xo).max()
+        print(f"  max |dx| {np.abs(xg - xo).max() / scale:.3e}  e {np.abs(Eg - E).max():.3e}")
+        assert np.abs(xg - xo).max() <= 1e-4 * scale
+        assert np.abs(Eg - E).max() <= 1e-4 * np.abs(E).max()
+        assert np.abs(mag_g - mag_o).max() <= 1e-4 * max(1e-12, np.abs(mag_o).max())
+        assert abs(tot_g - tot_o) <= 1e-4 * max(1e-12, abs(tot_o))
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)

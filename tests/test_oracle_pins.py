@@ -1,0 +1,448 @@
+"""CPU tests: the oracle pinned against the reference's golden vectors, against
+the reference's own buildable sources (oracle/_ref) and against closed forms.
+
+Each test names what pins the function; a plausible mistake (dropped term,
+wrong sign, swapped index) fails at least one of them.
+"""
+import ctypes as C
+import dataclasses
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, SMALL4D, TINY, make_scan, oracle_path
+from timbir_python_amd import synth, xt_abi
+import py_oracle
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+needs_ref = pytest.mark.skipif(not py_oracle.RefLib.available(),
+                               reason="oracle/_ref not built (reference sources absent)")
+
+
+# -- random numbers ----------------------------------------------------------
+def test_rng_known_answer_table(oracle32):
+    """Park-Miller states from seed 1: the table in randlib.c:69-91."""
+    gold = json.load(open(os.path.join(GOLD, "randlib_seed1.json")))
+    out = np.zeros(20)
+    oracle32.lib.oracle_random2_seq(1, 20, out.ctypes.data)
+    states = np.rint(out * 2147483647).astype(np.int64)
+    assert states.tolist() == gold["states"]
+
+
+@needs_ref
+def test_rng_bitexact_vs_reference(oracle32):
+    ref = py_oracle.RefLib(False)
+    n = 20000
+    out = np.zeros(n)
+    oracle32.lib.oracle_random2_seq(761521, n, out.ctypes.data)
+    ref.lib.srandom2(761521)
+    r = np.array([ref.lib.random2() for _ in range(n)])
+    assert np.array_equal(out, r)
+
+
+# -- view order and view files ----------------------------------------------
+def test_interlaced_views_golden_log(oracle32):
+    """Bit-reversed interlaced order + clipping: the 399 deleted views and their
+    times of the launcher log info.txt:1-175 (K=32, N_theta=1536, 25 cycles,
+    any clip threshold in [2.8125, 3.75) degrees)."""
+    gold = json.load(open(os.path.join(GOLD, "interlaced_clip.json")))
+    K, Nt = gold["K"], gold["N_theta"]
+    Np = Nt * 25
+    views, times = np.zeros(Np), np.zeros(Np)
+    dv, dt = np.zeros(Np), np.zeros(Np)
+    nd = C.c_int32()
+    kept = oracle32.lib.oracle_interlaced_views(K, Nt, Np, 0.03, 100.0, views.ctypes.data,
+                                                times.ctypes.data, dv.ctypes.data, dt.ctypes.data,
+                                                C.byref(nd))
+    assert nd.value == 399 and kept == Np - 399
+    assert np.array_equal(dv[:399], np.array(gold["deleted_views"]))
+    assert np.array_equal(dt[:399], np.array(gold["deleted_times"]))
+    # the hand check of one entry: 360.9375 deg = index 3080 = 96*32 + bitrev5(2)
+    assert dv[0] == 3080 * 180.0 / Nt and dt[0] == 3080 / 32
+
+
+def test_synth_interlaced_views_match_oracle(oracle32):
+    cfg = dataclasses.replace(synth.C3, N_p=1000)
+    ang, t = cfg.views()
+    Np = cfg.N_p
+    views, times = np.zeros(Np), np.zeros(Np)
+    dv, dt = np.zeros(Np), np.zeros(Np)
+    nd = C.c_int32()
+    oracle32.lib.oracle_interlaced_views(cfg.interlaced_K, cfg.interlaced_N_theta, Np, 0.0, 0.0,
+                                         views.ctypes.data, times.ctypes.data, dv.ctypes.data,
+                                         dt.ctypes.data, C.byref(nd))
+    assert nd.value == 0
+    assert np.array_equal(ang, views) and np.array_equal(t[1:], times[1:])
+
+
+def test_parse_reference_view_info(oracle32):
+    """The view_info.txt data file shipped with the engine: 2000 '<time> - <deg>,' lines."""
+    path = os.path.join(GOLD, "view_info_reference.txt")
+    views = np.zeros(2000, np.float32)
+    times = np.zeros(2000, np.float32)
+    n = oracle32.lib.oracle_parse_view_info(path.encode(), 2000, views.ctypes.data,
+                                            times.ctypes.data)
+    assert n == 2000
+    lines = open(path).read().split("\n")
+    t0, a0 = lines[1].split(" - ")
+    assert times[1] == np.float32(float(t0))
+    # (Real_t)atof(s) * M_PI / 180.0: float operand promoted to double
+    assert views[1] == np.float32(float(np.float32(float(a0.rstrip(",")))) * math.pi / 180.0)
+    assert np.all(np.diff(views) > 0)
+
+
+def test_parse_view_info_rejects_bad_format(oracle32, tmp_path):
+    p = tmp_path / "view_info.txt"
+    p.write_text("0.0 - 1.0,\n1.0 ; 2.0,\n")
+    v = np.zeros(4, np.float32)
+    assert oracle32.lib.oracle_parse_view_info(str(p).encode(), 4, v.ctypes.data,
+                                               v.ctypes.data) == -1
+
+
+def test_bin_views_closed_form(oracle32):
+    """Frames of width delta starting at Rtime0; times past the last frame stay in it."""
+    t = np.sort(np.random.default_rng(0).uniform(0, 10, 200)).astype(np.float32)
+    idx = np.zeros(4 * 200, np.int32)
+    num = np.zeros(4, np.int32)
+    oracle32.lib.oracle_bin_views(t.ctypes.data, 200, 4, C.c_float(0.0), C.c_float(2.0),
+                                  idx.ctypes.data, num.ctypes.data)
+    expect = np.minimum(np.floor(t / 2.0).astype(int), 3)
+    for f in range(4):
+        assert np.array_equal(idx[f * 200: f * 200 + num[f]], np.nonzero(expect == f)[0])
+
+
+# -- detector table and system matrix --------------------------------------
+def _setup(oracle, cfg, scan_root):
+    path, info = make_scan(cfg, scan_root)
+    path = oracle_path(path, oracle)
+    S, O, T = oracle.setup(path, cfg.cli_args())
+    oracle.read_sinogram(path, S, T)
+    return path, info, (S, O, T)
+
+
+@needs_ref
+@pytest.mark.parametrize("double", [False, True])
+def test_detector_response_bitexact_vs_reference(double, scan_root):
+    o = py_oracle.Oracle(double)
+    ref = py_oracle.RefLib(double)
+    path, info, (S, O, T) = _setup(o, SMALL4D, scan_root)
+    try:
+        H = o.detector_response(S, O, T)
+        Hr = np.zeros_like(H)
+        ht = xt_abi.PtrTree(Hr)
+        Ht = np.zeros(65, H.dtype)
+        ref.lib.DetectorResponseProfile(ht.ptr, C.c_void_p(Ht.ctypes.data), C.byref(S), C.byref(O),
+                                        C.byref(T))
+        assert np.array_equal(H, Hr)
+    finally:
+        o.free(S, O, T)
+
+
+def test_detector_response_closed_forms(oracle64, scan_root):
+    """H_r[.][64] = 0; theta and theta+90 deg give the same row; the table is
+    the trapezoid (unit-area beam) so H_r[0] equals the trapezoid's plateau
+    value delta_xy/cos(theta) near theta = 0 up to the beam smoothing."""
+    cfg = dataclasses.replace(TINY, N_p=64)
+    path, info, (S, O, T) = _setup(oracle64, cfg, scan_root)
+    try:
+        H = oracle64.detector_response(S, O, T)
+        assert np.all(H[:, 64] == 0)
+        # views are i*180/64: view i+32 is i + 90 degrees
+        assert np.allclose(H[:32], H[32:], rtol=1e-12, atol=1e-15)
+        assert abs(H[0, 0] - O.delta_xy) < 1e-9  # beam inside the plateau at theta=0
+        assert np.all(np.diff(H[:, :64], axis=1) <= 1e-12)  # non-increasing with distance
+    finally:
+        oracle64.free(S, O, T)
+
+
+@needs_ref
+def test_amatrix_bitexact_vs_reference(oracle32, scan_root):
+    ref = py_oracle.RefLib(False)
+    path, info, (S, O, T) = _setup(oracle32, TINY, scan_root)
+    try:
+        H = oracle32.detector_response(S, O, T)
+        ht = xt_abi.PtrTree(H)
+        cols = []
+        for lib in (oracle32.lib.oracle_amatrix_column, ref.lib.calcAMatrixColumnforAngle):
+            vals = np.zeros(8, np.float32)
+            inds = np.zeros(8, np.int32)
+            a = oracle32.AMatrixCol()
+            a.values, a.index = vals.ctypes.data, inds.ctypes.data
+            out = []
+            for v in range(0, S.N_p, 5):
+                for yy in range(O.N_y):
+                    for xx in range(0, O.N_x, 3):
+                        lib(C.byref(S), C.byref(O), ht.ptr, C.byref(a), yy, xx, v)
+                        out.append((a.count, inds[:a.count].tolist(),
+                                    vals[:a.count].view(np.uint32).tolist()))
+            cols.append(out)
+        assert cols[0] == cols[1]
+    finally:
+        oracle32.free(S, O, T)
+
+
+def test_amatrix_area_invariant(oracle64, scan_root):
+    """The footprint integrates to the voxel's area: the table integral over
+    +-64*OffsetR equals delta_xy^2 for every view, and the column sum, averaged
+    over voxels (sub-bin phases), equals delta_xy^2 / delta_r.  A single
+    column's sum aliases with its phase (the bins sample a ~2-bin footprint)."""
+    path, info, (S, O, T) = _setup(oracle64, TINY, scan_root)
+    try:
+        H = oracle64.detector_response(S, O, T)
+        d = np.arange(65) * S.OffsetR
+        area = O.delta_xy ** 2
+        for v in range(S.N_p):
+            assert 2 * np.trapezoid(H[v], d) == pytest.approx(area, rel=3e-3)
+        ht = xt_abi.PtrTree(H)
+        vals = np.zeros(8, np.float64)
+        inds = np.zeros(8, np.int32)
+        a = oracle64.AMatrixCol()
+        a.values, a.index = vals.ctypes.data, inds.ctypes.data
+        for v in range(0, S.N_p, 7):
+            sums = []
+            for yy in range(4, O.N_y - 4):
+                for xx in range(4, O.N_x - 4):
+                    oracle64.lib.oracle_amatrix_column(C.byref(S), C.byref(O), ht.ptr, C.byref(a),
+                                                       yy, xx, v)
+                    sums.append(vals[:a.count].sum())
+            assert np.mean(sums) == pytest.approx(area / S.delta_r, rel=0.01)
+    finally:
+        oracle64.free(S, O, T)
+
+
+@needs_ref
+def test_forward_project_vs_reference_scatter(oracle32, scan_root):
+    """Ax of the oracle = the reference's forward_project_voxel summed over every
+    voxel in the same order (F/XT_ICD_update.c:552-569), bit for bit."""
+    ref = py_oracle.RefLib(False)
+    path, info, (S, O, T) = _setup(oracle32, TINY, scan_root)
+    try:
+        vol = info["phantom"].astype(np.float32)
+        oracle32.set_object(O, vol)
+        H = oracle32.detector_response(S, O, T)
+        ax = oracle32.forward_project(S, O, H)
+        ht = xt_abi.PtrTree(H)
+        E = np.zeros_like(ax)
+        et = xt_abi.PtrTree(E)
+        vals = np.zeros(8, np.float32)
+        inds = np.zeros(8, np.int32)
+        a = oracle32.AMatrixCol()
+        a.values, a.index = vals.ctypes.data, inds.ctypes.data
+        lists, num = oracle32.proj_lists(O)
+        for yy in range(O.N_y):
+            for xx in range(O.N_x):
+                for p in lists[0]:
+                    ref.lib.calcAMatrixColumnforAngle(C.byref(S), C.byref(O), ht.ptr, C.byref(a),
+                                                      yy, xx, int(p))
+                    for z in range(O.N_z):
+                        ref.lib.forward_project_voxel(C.byref(S), C.c_float(vol[0, z, yy, xx]),
+                                                      et.ptr, C.byref(a), int(p), z)
+        assert np.array_equal(ax, E)
+    finally:
+        oracle32.free(S, O, T)
+
+
+# -- NHICD selection ---------------------------------------------------------
+@needs_ref
+def test_vsc_select_bitexact_vs_reference(oracle32, scan_root):
+    """Same line sets and the same random numbers consumed (F/XT_NHICD.c:164-216)."""
+    ref = py_oracle.RefLib(False)
+    path, info, (S, O, T) = _setup(oracle32, SMALL4D, scan_root)
+    try:
+        nb, nxy = T.num_z_blocks, O.N_y * O.N_x
+        rng = np.random.default_rng(3)
+        mag = rng.exponential(1.0, (O.N_time, nb, O.N_y, O.N_x)).astype(np.float32)
+        mag[:, :, :2] = 0
+        usn = py_oracle.deref_data(T.UpdateSelectNum, 1, C.c_int32, O.N_time * nb)
+        usn[:] = rng.integers(nxy // 2, nxy + 1, usn.size)
+        results = []
+        for fn, seed in ((oracle32.lib.oracle_vsc_select, oracle32.lib.oracle_srandom2),
+                         (ref.lib.VSC_based_Voxel_Line_Select, ref.lib.srandom2)):
+            seed(4242)
+            mt = xt_abi.PtrTree(mag.copy())
+            fn(C.byref(O), C.byref(T), mt.ptr)
+            num = py_oracle.deref_data(T.NHICDSelectNum, 1, C.c_int32, O.N_time * nb).copy()
+            xs = py_oracle.deref_data(T.x_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).copy()
+            ys = py_oracle.deref_data(T.y_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).copy()
+            nxt = (oracle32.lib.oracle_random2() if fn is oracle32.lib.oracle_vsc_select
+                   else ref.lib.random2())
+            sel = [(num[k], xs.reshape(-1, nxy)[k, :num[k]].tolist(),
+                    ys.reshape(-1, nxy)[k, :num[k]].tolist()) for k in range(O.N_time * nb)]
+            results.append((sel, nxt))
+        assert results[0] == results[1]
+    finally:
+        oracle32.free(S, O, T)
+
+
+# -- voxel update: closed form of the quadratic case -------------------------
+QUAD = dataclasses.replace(synth.C1, name="quad", N_xy=6, N_z=2, N_p=12, p=2.0, c_s=1.0,
+                           sigma_s=2e-3, alpha=1.0, zinger_T=1e9, zinger_frac=0.0,
+                           update_offset=0, zero_mean_offset=False, iterations=400,
+                           num_threads=2, n_ellipsoids=3, seed=2)
+
+
+def dense_map_solution(oracle64, path, cfg):
+    """Minimiser of 1/2 (y-Ax)'W(y-Ax) + sum_pairs b (x_j-x_k)^2/(sigma^2 (c+1)),
+    the MAP cost when p = q = 2, by a dense solve."""
+    S, O, T = oracle64.setup(path, cfg.cli_args())
+    try:
+        oracle64.read_sinogram(path, S, T)
+        H = oracle64.detector_response(S, O, T)
+        ht = xt_abi.PtrTree(H)
+        nz, ny, nx = O.N_z, O.N_y, O.N_x
+        nvox = nz * ny * nx
+        nsino = S.N_p * S.N_r * S.N_t
+        A = np.zeros((nsino, nvox))
+        vals = np.zeros(8)
+        inds = np.zeros(8, np.int32)
+        a = oracle64.AMatrixCol()
+        a.values, a.index = vals.ctypes.data, inds.ctypes.data
+        for yy in range(ny):
+            for xx in range(nx):
+                for p in range(S.N_p):
+                    oracle64.lib.oracle_amatrix_column(C.byref(S), C.byref(O), ht.ptr, C.byref(a),
+                                                       yy, xx, p)
+                    for q in range(a.count):
+                        for z in range(nz):
+                            A[(p * S.N_r + inds[q]) * S.N_t + z, (z * ny + yy) * nx + xx] = vals[q]
+        y = oracle64.sino_view(S.Projection, S).reshape(-1).copy()
+        w = oracle64.sino_view(T.Weight, S).reshape(-1).copy()
+        filt = np.array(T.Spatial_Filter).reshape(27)
+        kappa = 2.0 / (T.Sigma_S_Q * (O.C_S + 1.0))
+        L = np.zeros((nvox, nvox))
+        for z in range(nz):
+            for yy in range(ny):
+                for xx in range(nx):
+                    i = (z * ny + yy) * nx + xx
+                    for dz in (-1, 0, 1):
+                        for dy in (-1, 0, 1):
+                            for dx in (-1, 0, 1):
+                                z2, y2, x2 = z + dz, yy + dy, xx + dx
+                                if (dz, dy, dx) == (0, 0, 0) or not (
+                                        0 <= z2 < nz and 0 <= y2 < ny and 0 <= x2 < nx):
+                                    continue
+                                k = (z2 * ny + y2) * nx + x2
+                                b = filt[((dz + 1) * 3 + dy + 1) * 3 + dx + 1] * kappa
+                                L[i, i] += b
+                                L[i, k] -= b
+        M = A.T @ (w[:, None] * A) + L
+        x = np.linalg.solve(M, A.T @ (w * y))
+        return x.reshape(nz, ny, nx)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def test_icd_converges_to_dense_map_solution(oracle64, scan_root):
+    """With p = q = 2, no zinger rejection and alpha = 1, ICD is Gauss-Seidel on
+    the normal equations: the whole driver (theta sums, surrogate, prior weights,
+    e updates, NHICD schedule) must land on the dense solution."""
+    path, info = make_scan(QUAD, scan_root)
+    path = oracle_path(path, oracle64)
+    xs = dense_map_solution(oracle64, path, QUAD)
+    oracle64.run(path, QUAD.cli_args())
+    x = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float64).reshape(xs.shape)
+    assert np.abs(x - xs).max() <= 1e-6 * np.abs(xs).max()
+
+
+def test_surrogate_matches_potential_derivative(oracle64, scan_root):
+    """b' = b * rho'(delta)/delta inside the update equals the finite-difference
+    derivative of the qGGMRF potential rho (F/XT_ICD_update.c:79-82)."""
+    path, info, (S, O, T) = _setup(oracle64, TINY, scan_root)
+    try:
+        p, c = O.MRF_P, O.C_S
+        sq, sqp = T.Sigma_S_Q, T.Sigma_S_Q_P
+        rho = lambda d: (abs(d) ** 2 / sq) / (c + abs(d) ** (2 - p) / sqp)
+        flt = np.zeros(27, np.uint8)
+        flt[0] = 1
+        nb = np.zeros(27)
+        tn = np.zeros(2)
+        tf = np.zeros(2, np.uint8)
+        b0 = T.Spatial_Filter[0][0][0]
+        for delta in [3e-6, -2e-5, 1e-4, -7e-4, 2e-3]:
+            V = 1e-3
+            nb[0] = V - delta
+            u = oracle64.lib.oracle_functional_substitution(
+                V, 0.0, 1.0, C.byref(O), C.byref(T), nb.ctypes.data, flt.ctypes.data,
+                tn.ctypes.data, tf.ctypes.data, 0)
+            bprime = (u - V) / (nb[0] - u)  # alpha = T.alpha
+            bprime = (T.alpha * (nb[0] - V) - (u - V)) and bprime
+            h = abs(delta) * 1e-4
+            drho = (rho(delta + h) - rho(delta - h)) / (2 * h)
+            # u = V + alpha*(u* - V), u* = (b' x_n + V)/(b' + 1)
+            ustar = V + (u - V) / T.alpha
+            b_est = (ustar - V) / (nb[0] - ustar)
+            assert b_est == pytest.approx(b0 * drho / delta, rel=1e-5)
+    finally:
+        oracle64.free(S, O, T)
+
+
+# -- offset, variance, cost: closed forms ------------------------------------
+def _rand_sino(rng, S):
+    shape = (S.N_p, S.N_r, S.N_t)
+    return rng.normal(0, 0.05, shape), rng.uniform(1e3, 2e4, shape)
+
+
+def test_offset_update_closed_forms(oracle64, scan_root):
+    """All entries selected: the update makes the weighted view-mean of e zero at
+    every detector pixel; with the zero-mean constraint the offsets sum to 0."""
+    path, info, (S, O, T) = _setup(oracle64, TINY, scan_root)
+    try:
+        rng = np.random.default_rng(1)
+        E0, W = _rand_sino(rng, S)
+        oracle64.sino_view(T.Weight, S)[:] = W
+        T.ErrorSinoThresh = 1e9
+        for zero_mean in (0, 1):
+            T.EnforceZeroMeanOffset = zero_mean
+            oracle64.offset_view(S)[:] = 0
+            E = E0.copy()
+            et = xt_abi.PtrTree(E)
+            oracle64.lib.oracle_offset_update(C.byref(S), C.byref(T), et.ptr)
+            d = oracle64.offset_view(S)
+            if zero_mean:
+                assert abs(d.sum()) <= 1e-9 * np.abs(d).sum()
+            else:
+                assert np.abs((E * W).sum(0) / W.sum(0)).max() <= 1e-12
+                assert np.allclose(d, (E0 * W).sum(0) / W.sum(0), rtol=1e-12, atol=1e-15)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def test_variance_update_closed_form(oracle64, scan_root):
+    path, info, (S, O, T) = _setup(oracle64, TINY, scan_root)
+    try:
+        rng = np.random.default_rng(2)
+        E, W = _rand_sino(rng, S)
+        oracle64.sino_view(T.Weight, S)[:] = W
+        T.ErrorSinoThresh = 1e9
+        T.var_est = 2.0
+        et = xt_abi.PtrTree(E)
+        oracle64.lib.oracle_variance_update(C.byref(S), C.byref(T), et.ptr)
+        expect = np.mean(E * E * W * 2.0)
+        assert T.var_est == pytest.approx(expect, rel=1e-12)
+        assert np.allclose(oracle64.sino_view(T.Weight, S), W * 2.0 / expect, rtol=1e-12)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def test_cost_closed_form(oracle64, scan_root):
+    """Zero object: prior 0; data term = sum of rho_huber over entries / 2 + log term."""
+    path, info, (S, O, T) = _setup(oracle64, TINY, scan_root)
+    try:
+        rng = np.random.default_rng(3)
+        E, W = _rand_sino(rng, S)
+        oracle64.sino_view(T.Weight, S)[:] = W
+        Tt, dl = T.ErrorSinoThresh, T.ErrorSinoDelta
+        sel = np.abs(E * np.sqrt(W)) < Tt
+        oracle64.sino_view(S.ProjSelect, S, bool)[:] = sel
+        oracle64.set_object(O, np.zeros((O.N_time, O.N_z, O.N_y, O.N_x)))
+        et = xt_abi.PtrTree(E)
+        c = oracle64.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        u = E * np.sqrt(W)
+        rho = np.where(sel, u * u, 2 * dl * Tt * np.abs(u) + Tt * Tt * (1 - 2 * dl))
+        expect = rho.sum() / 2 + E.size * math.log(T.var_est) / 2
+        assert c == pytest.approx(expect, rel=1e-12)
+    finally:
+        oracle64.free(S, O, T)

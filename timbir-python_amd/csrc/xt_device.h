@@ -1,0 +1,921 @@
+// xt_device.h -- gfx950 kernels of the ICD path.
+//
+// Layout in HBM (one GPU holds the whole slab):
+//   e, w, y     float [N_p][N_r][N_t]     error sinogram, weights, measurements
+//                                         (t fastest: the slices of a voxel column
+//                                         are contiguous, one 128 B line per
+//                                         (view, bin) at N_t = 32)
+//   d           float [N_r][N_t]          ring offsets
+//   x           float [N_time][N_y][N_x][N_z]   object, z fastest (a column is one
+//                                         contiguous run; wave lanes map to z)
+//   Hr          float [N_p][65]           detector response table
+//   mag_blk     float [N_time][nb][N_y][N_x]  per-z-block |update| of each line
+//   magsum      float [N_time][N_y][N_x]  the same summed over blocks
+//
+// Reference arithmetic is reproduced where it is integer-like (bin indices of a
+// system-matrix column, the detector table, the NHICD filter): those functions
+// disable FMA contraction and follow the reference's float/double steps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define XT_MAXA 6          // max detector bins of one A column (2*delta_xy/delta_r + 2)
+#define XT_NBINS 64        // DETECTOR_RESPONSE_BINS
+#define XT_HR_STRIDE 65
+#define XT_PI 3.14159265358979323846
+#define XT_PI_2 1.57079632679489661923132169163975144
+#define XT_PI_4 0.785398163397448309615660845819875721
+
+struct XtGeom {
+  int N_p, N_r, N_t, N_x, N_y, N_z, N_time, zo;
+  int nb;          // z blocks of the magnitude map
+  int max_views;   // max views of a frame
+  float delta_r, delta_t, R0, RMax, OffsetR;
+  float x0, y0, delta_xy;
+  // prior (F/XT_Init.c:199-256,391-394)
+  float MRF_P, C_S, C_T, Sigma_S_Q, Sigma_S_Q_P, Sigma_T_Q, Sigma_T_Q_P;
+  float filt[27];  // Spatial_Filter, [dz][dy][dx] flattened
+  float tfilt;     // Time_Filter[0]
+  float alpha, T, delta;  // over-relaxation, generalized-Huber T and delta
+  int positivity;
+};
+
+// ---------------------------------------------------------------------------
+// System-matrix column for (row, col, view): the exact float steps of
+// F/XT_AMatrix.c:50-110.  Writes the first candidate bin and XT_MAXA values
+// (0 for bins that are not part of the column); returns the entry count.
+__device__ __forceinline__ int xt_amatrix_column(const XtGeom &g, const float *__restrict__ H,
+                                                 float cosv, float sinv, int row, int col,
+                                                 int *first, float vals[XT_MAXA]) {
+#pragma clang fp contract(off)
+  float x = (float)((double)g.x0 + ((double)(float)col + 0.5) * (double)g.delta_xy);
+  float y = (float)((double)g.y0 + ((double)(float)row + 0.5) * (double)g.delta_xy);
+  float r = x * cosv - y * sinv;
+  float rmin = r - g.delta_xy;
+  float rmax = r + g.delta_xy;
+#pragma unroll
+  for (int b = 0; b < XT_MAXA; b++) vals[b] = 0.f;
+  *first = 0;
+  if (rmax < g.R0 || rmin > g.RMax) return 0;
+  int imin = (int)floorf((rmin - g.R0) / g.delta_r);
+  int imax = (int)floorf((rmax - g.R0) / g.delta_r);
+  if (imax >= g.N_r) imax = g.N_r - 1;
+  if (imin < 0) imin = 0;
+  *first = imin;
+  int count = 0;
+#pragma unroll
+  for (int b = 0; b < XT_MAXA; b++) {
+    int j = imin + b;
+    if (j <= imax) {
+      float Rc = (float)((double)g.R0 + ((double)(float)j + 0.5) * (double)g.delta_r);
+      float dr = fabsf(r - Rc);
+      int k = (int)floorf(dr / g.OffsetR);
+      if (k >= 0 && k < XT_NBINS) {
+        float w1 = dr - (float)k * g.OffsetR;
+        float w2 = g.OffsetR - w1;
+        float f1 = (w2 * H[k] + w1 * H[k + 1]) / g.OffsetR;
+        if (f1 > 0) {
+          vals[b] = f1;
+          count++;
+        }
+      }
+    }
+  }
+  return count;
+}
+
+// A entry of bin j for voxel (row, col) and one view, or 0 when bin j is not
+// in that column (same steps as xt_amatrix_column, for the ray-driven gather).
+__device__ __forceinline__ float xt_amatrix_entry(const XtGeom &g, const float *__restrict__ H,
+                                                  float cosv, float sinv, int row, int col, int j) {
+#pragma clang fp contract(off)
+  float x = (float)((double)g.x0 + ((double)(float)col + 0.5) * (double)g.delta_xy);
+  float y = (float)((double)g.y0 + ((double)(float)row + 0.5) * (double)g.delta_xy);
+  float r = x * cosv - y * sinv;
+  float rmin = r - g.delta_xy;
+  float rmax = r + g.delta_xy;
+  if (rmax < g.R0 || rmin > g.RMax) return 0.f;
+  int imin = (int)floorf((rmin - g.R0) / g.delta_r);
+  int imax = (int)floorf((rmax - g.R0) / g.delta_r);
+  if (imax >= g.N_r) imax = g.N_r - 1;
+  if (imin < 0) imin = 0;
+  if (j < imin || j > imax) return 0.f;
+  float Rc = (float)((double)g.R0 + ((double)(float)j + 0.5) * (double)g.delta_r);
+  float dr = fabsf(r - Rc);
+  int k = (int)floorf(dr / g.OffsetR);
+  if (k < 0 || k >= XT_NBINS) return 0.f;
+  float w1 = dr - (float)k * g.OffsetR;
+  float w2 = g.OffsetR - w1;
+  float f1 = (w2 * H[k] + w1 * H[k + 1]) / g.OffsetR;
+  return f1 > 0 ? f1 : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Detector response table, one workgroup per view: the trapezoidal voxel
+// profile (F/XT_Profile.c:46-103) convolved with the unit-area Hamming beam
+// (F/XT_Profile.c:107-123), sampled at i*OffsetR (F/XT_Profile.c:128-173).
+__global__ void __launch_bounds__(256) k_detector_response(XtGeom g, const float *__restrict__ views,
+                                                           float beam_width, float *__restrict__ Hr) {
+#pragma clang fp contract(off)
+  __shared__ float prof[1536];
+  __shared__ float beam[512];
+  __shared__ float beam_sum;
+  const int k = blockIdx.x, tid = threadIdx.x;
+  float angle = views[k];
+  while ((double)angle > XT_PI_2) angle = (float)((double)angle - XT_PI_2);
+  while (angle < 0) angle = (float)((double)angle + XT_PI_2);
+  float MaxVal;
+  if ((double)angle <= XT_PI_4)
+    MaxVal = (float)((double)g.delta_xy / cos((double)angle));
+  else
+    MaxVal = (float)((double)g.delta_xy / cos(XT_PI_2 - (double)angle));
+  float temp = (float)cos(XT_PI_4);
+  float dist1 = (float)((double)temp * cos(XT_PI_4 - (double)angle));
+  float dist2 = (float)((double)temp * fabs(cos(XT_PI_4 + (double)angle)));
+  float LeftCorner = 1 - dist1, LeftNear = 1 - dist2, RightNear = 1 + dist2, RightCorner = 1 + dist1;
+  for (int j = tid; j < 1536; j += blockDim.x) {
+    float t = (float)(2.0 * j / 1536);
+    float v;
+    if (t <= LeftCorner || t >= RightCorner)
+      v = 0;
+    else if (t > RightNear)
+      v = MaxVal * (RightCorner - t) / (RightCorner - RightNear);
+    else if (t >= LeftNear)
+      v = MaxVal;
+    else
+      v = MaxVal * (t - LeftCorner) / (LeftNear - LeftCorner);
+    prof[j] = v;
+  }
+  for (int i = tid; i < 512; i += blockDim.x)
+    beam[i] = (float)(0.54 - 0.46 * cos((2 * XT_PI / 512) * i));
+  __syncthreads();
+  if (tid == 0) {
+    float s = 0;
+    for (int i = 0; i < 512; i++) s = s + beam[i];
+    beam_sum = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < 512; i += blockDim.x) beam[i] /= beam_sum;
+  __syncthreads();
+  if (tid < XT_NBINS) {
+    float r0 = -(beam_width) / 2;
+    float StepSize = beam_width / 512;
+    float TempConst = 1536 / (2 * g.delta_xy);
+    float ProfileCenterR = tid * g.OffsetR;
+    float rmin = ProfileCenterR - g.delta_xy;
+    float sum = 0;
+    for (int p = 0; p < 512; p++) {
+      float r = r0 + p * StepSize;
+      int idx = (int)floorf((r - rmin) * TempConst);
+      if (idx >= 0 && idx <= 1535) sum += prof[idx] * beam[p];
+    }
+    Hr[(size_t)k * XT_HR_STRIDE + tid] = sum;
+  } else if (tid == XT_NBINS) {
+    Hr[(size_t)k * XT_HR_STRIDE + XT_NBINS] = 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection by rays: one thread per (view, bin, slice), FP64 sum over
+// every voxel whose A column for that view holds the bin.  Candidates are
+// enumerated along the axis more perpendicular to the ray (at most ~6 per
+// row or column); each is accepted by the exact column test.
+// mode 0: out64 = Ax.   mode 1: e = y - Ax - d (float).
+template <int ZL>
+__global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *__restrict__ x,
+                                                         const float *__restrict__ Hr,
+                                                         const float *__restrict__ cosv,
+                                                         const float *__restrict__ sinv,
+                                                         const int *__restrict__ view_frame,
+                                                         int mode, double *__restrict__ out64,
+                                                         const float *__restrict__ y,
+                                                         const float *__restrict__ d,
+                                                         float *__restrict__ e) {
+  constexpr int RB = 256 / ZL;
+  const int v = blockIdx.y;
+  const int zl = threadIdx.x % ZL, jb = threadIdx.x / ZL;
+  const int j = blockIdx.x * RB + jb;
+  const bool lane_ok = (j < g.N_r) && (zl < g.N_z);
+  const int f = view_frame[v];
+  double acc = 0.0;
+  if (j < g.N_r && f >= 0) {
+    const float c = cosv[v], s = sinv[v];
+    const float *H = Hr + (size_t)v * XT_HR_STRIDE;
+    const float *xf = x + (size_t)f * g.N_y * g.N_x * g.N_z;
+    // accepted voxels satisfy |r - Rc| <= delta_xy + delta_r/2; widen slightly
+    const double Rc = (double)g.R0 + ((double)j + 0.5) * (double)g.delta_r;
+    const double W = (double)g.delta_xy + 0.5 * (double)g.delta_r + 1e-3 * (double)g.delta_r;
+    const double cd = c, sd = s, dxy = g.delta_xy;
+    if (fabs(cd) >= fabs(sd)) {
+      for (int row = 0; row < g.N_y; row++) {
+        double yc = (double)g.y0 + (row + 0.5) * dxy;
+        double xa = (Rc - W + yc * sd) / cd, xb = (Rc + W + yc * sd) / cd;
+        if (xa > xb) { double tmp = xa; xa = xb; xb = tmp; }
+        int c0 = (int)floor((xa - (double)g.x0) / dxy - 0.5);
+        int c1 = (int)ceil((xb - (double)g.x0) / dxy - 0.5);
+        if (c0 < 0) c0 = 0;
+        if (c1 > g.N_x - 1) c1 = g.N_x - 1;
+        for (int col = c0; col <= c1; col++) {
+          float a = xt_amatrix_entry(g, H, c, s, row, col, j);
+          if (a != 0.f && lane_ok)
+            acc += (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl] * (double)a;
+        }
+      }
+    } else {
+      for (int col = 0; col < g.N_x; col++) {
+        double xc = (double)g.x0 + (col + 0.5) * dxy;
+        double ya = (xc * cd - (Rc + W)) / sd, yb = (xc * cd - (Rc - W)) / sd;
+        if (ya > yb) { double tmp = ya; ya = yb; yb = tmp; }
+        int r0 = (int)floor((ya - (double)g.y0) / dxy - 0.5);
+        int r1 = (int)ceil((yb - (double)g.y0) / dxy - 0.5);
+        if (r0 < 0) r0 = 0;
+        if (r1 > g.N_y - 1) r1 = g.N_y - 1;
+        for (int row = r0; row <= r1; row++) {
+          float a = xt_amatrix_entry(g, H, c, s, row, col, j);
+          if (a != 0.f && lane_ok)
+            acc += (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl] * (double)a;
+        }
+      }
+    }
+  }
+  if (!lane_ok) return;
+  // z_overlap_num == 1: slice zl is detector row t = zl
+  const size_t idx = ((size_t)v * g.N_r + j) * g.N_t + zl;
+  if (mode == 0) {
+    out64[idx] = acc;
+  } else {
+    double ev = (double)y[idx] - acc - (double)d[(size_t)j * g.N_t + zl];
+    e[idx] = (float)ev;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// qGGMRF surrogate weight b' = b * rho'(delta)/delta (F/XT_ICD_update.c:89-125,
+// 141-168), float.
+__device__ __forceinline__ float xt_qggmrf_ratio(float delta, float p, float c, float sq,
+                                                 float sqp) {
+  if (delta == 0.f) return 2.0f / (sq * c);
+  float ad = fabsf(delta);
+  float t1 = powf(ad, 2.0f - p) / sqp;
+  float t3 = c + t1;
+  // rho'(delta)/delta = |delta|^(q-2)/(t3*sq) * (q - (q-p)*t1/t3), q = 2
+  return (1.0f / (t3 * sq)) * (2.0f - ((2.0f - p) * t1) / t3);
+}
+
+// Per-launch work description of the voxel-update kernel.
+struct XtIcdLaunch {
+  int mode;        // 0 homogeneous sub-iteration, 1 NHICD lines
+  int phase;       // 0..3: frame f updates tiles of colour (phase + f) & 3
+  int sub_iter;    // j of K
+  int K;           // NHICD_Iterations
+  int iter;        // ICD iteration (1-based)
+  uint32_t seed;
+  int count_stats; // accumulate total_vox_mag (homogeneous sweeps)
+  int zorder;      // 0: slices in red-black order; 1: ascending, one slice at a time
+                   // (the reference's slice loop, F/XT_VoxUpdate.c:135; per-voxel checks)
+};
+
+struct XtIcdBuffers {
+  float *e;
+  const float *w;
+  float *x;
+  const float *Hr;
+  const float *cosv, *sinv;
+  const int *views;          // [N_time][max_views]
+  const int *proj_num;       // [N_time]
+  const int *tiles_by_color; // [4][max_tiles]
+  const int *n_tiles;        // [4]
+  int max_tiles;
+  const int *tile_start;     // [n_tiles_total + 1] into tile_cols
+  const int *tile_cols;      // packed y*N_x + x, masked columns in tile raster order
+  const uint8_t *nhicd_flag; // [N_time][N_y][N_x]
+  float *mag_blk;            // [N_time][nb][N_y][N_x]
+  float *magsum;             // [N_time][N_y][N_x]
+  double *stats;             // [0] total_vox_mag, [1] voxel updates, [2] footprint entries
+};
+
+__device__ __forceinline__ uint32_t xt_hash(uint32_t a) {
+  a ^= a >> 16;
+  a *= 0x7feb352dU;
+  a ^= a >> 15;
+  a *= 0x846ca68bU;
+  a ^= a >> 16;
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// Voxel-line update of one super-voxel tile per workgroup (F/XT_VoxUpdate.c:
+// 11-254 with the surrogate of F/XT_ICD_update.c:130-179).  The workgroup walks
+// its tile's lines one after another.  For each line, threads (view group grp,
+// slice zl) sweep the sinogram footprint for theta1/theta2 (the A column of
+// each view is built once and kept in LDS), wave 0 updates the slices in
+// red-black order (exact Gauss-Seidel: slices of one line share no sinogram
+// entries), and all threads subtract A*delta from e with no-return atomics.
+// Tiles of one colour are never adjacent, so concurrently updated lines are
+// not prior neighbours; the colour is shifted per frame so adjacent frames
+// never update the same line at once.
+template <int ZL>
+__global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+  constexpr int NT = 256;
+  constexpr int G = NT / ZL;
+  extern __shared__ float smem[];
+  float *a_vals = smem;                                            // [max_views][XT_MAXA]
+  int *a_first = (int *)(a_vals + (size_t)g.max_views * XT_MAXA);  // [max_views]
+  float *red1 = (float *)(a_first + g.max_views);                  // [NT]
+  float *red2 = red1 + NT;                                         // [NT]
+  float *xcol = red2 + NT;                                         // [ZL]
+  float *delta = xcol + ZL;                                        // [ZL]
+  float *magb = delta + ZL;                                        // [64]
+  int *order = (int *)(magb + 64);                                 // [256]
+  __shared__ int s_n_lines, s_cnt_col, s_perf;
+
+  const int tid = threadIdx.x;
+  const int zl = tid % ZL, grp = tid / ZL;
+  const int f = blockIdx.y;
+  const int color = (L.phase + f) & 3;
+  if ((int)blockIdx.x >= B.n_tiles[color]) return;
+  const int tile = B.tiles_by_color[color * B.max_tiles + blockIdx.x];
+  const int t0 = B.tile_start[tile], nt = B.tile_start[tile + 1] - t0;
+  const int nv = B.proj_num[f];
+  const int *vlist = B.views + (size_t)f * g.max_views;
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  const bool zok = zl < g.N_z;
+
+  // ---- lines of the tile this launch updates, in order
+  if (tid == 0) {
+    int n = 0;
+    if (L.mode == 0) {
+      // a random order of the tile's lines for this iteration (the same in every
+      // sub-iteration), then its j-th 1/K part as F/XT_ICD_update.c:807-809
+      for (int i = 0; i < nt; i++) order[i] = B.tile_cols[t0 + i];
+      uint32_t h = xt_hash(L.seed ^ xt_hash((uint32_t)L.iter * 0x9E3779B9U ^
+                                            xt_hash((uint32_t)f * 7919U + (uint32_t)tile)));
+      for (int i = nt - 1; i > 0; i--) {
+        h = xt_hash(h + 0x6D2B79F5U);
+        const int jx = (int)(h % (uint32_t)(i + 1));
+        const int tmp = order[i];
+        order[i] = order[jx];
+        order[jx] = tmp;
+      }
+      const int per = nt / L.K;
+      const int xs = L.sub_iter * per;
+      const int xe = (L.sub_iter == L.K - 1) ? nt - 1 : (L.sub_iter + 1) * per - 1;
+      for (int i = xs; i <= xe; i++) order[n++] = order[i];
+    } else {
+      for (int i = 0; i < nt; i++) {
+        const int c = B.tile_cols[t0 + i];
+        if (B.nhicd_flag[(size_t)f * NYX + c]) order[n++] = c;
+      }
+    }
+    s_n_lines = n;
+    s_cnt_col = 0;
+    s_perf = 0;
+  }
+  __syncthreads();
+  const int n_lines = s_n_lines;
+  float vox_mag = 0.f;     // lane-local sum of |new x| (wave 0)
+  double n_upd = 0.0, n_foot = 0.0;
+
+  for (int li = 0; li < n_lines; li++) {
+    const int c = order[li];
+    const int yy = c / g.N_x, xx = c - yy * g.N_x;
+    // earlier lines' e and x writes of this workgroup must be seen by its loads
+    __threadfence();
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // ---- theta pass over the footprint
+    float th1 = 0.f, th2 = 0.f;
+    int cnt_local = 0;
+    for (int p = grp; p < nv; p += G) {
+      const int v = vlist[p];
+      int first;
+      float vals[XT_MAXA];
+      const int cnt = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v],
+                                        yy, xx, &first, vals);
+      if (zl == 0) {
+        a_first[p] = first;
+#pragma unroll
+        for (int b = 0; b < XT_MAXA; b++) a_vals[(size_t)p * XT_MAXA + b] = vals[b];
+        cnt_local += cnt;
+      }
+      if (zok) {
+        const size_t base = ((size_t)v * g.N_r + first) * g.N_t + zl;
+        float ev[XT_MAXA], wv[XT_MAXA];
+#pragma unroll
+        for (int b = 0; b < XT_MAXA; b++) {
+          ev[b] = 0.f;
+          wv[b] = 0.f;
+          if (vals[b] > 0.f) {
+            ev[b] = B.e[base + (size_t)b * g.N_t];
+            wv[b] = B.w[base + (size_t)b * g.N_t];
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < XT_MAXA; b++) {
+          const float pe = vals[b] * g.delta_t;
+          // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
+          if (ev[b] * ev[b] * wv[b] < g.T * g.T) {
+            th2 += pe * pe * wv[b];
+            th1 += -(ev[b] * pe * wv[b]);
+          } else {
+            const float ts = g.T * g.delta * sqrtf(wv[b]) / fabsf(ev[b]);
+            th2 += pe * pe * ts;
+            th1 += -(ev[b] * pe * ts);
+          }
+        }
+      }
+    }
+    red1[tid] = th1;
+    red2[tid] = th2;
+    if (zl == 0 && cnt_local) atomicAdd(&s_cnt_col, cnt_local);
+    __syncthreads();
+
+    // ---- theta reduction and the line's current values (wave 0)
+    if (tid < ZL) {
+      float t1 = 0.f, t2 = 0.f;
+      for (int q = 0; q < G; q++) {
+        t1 += red1[q * ZL + tid];
+        t2 += red2[q * ZL + tid];
+      }
+      red1[tid] = t1;
+      red2[tid] = t2;
+      xcol[tid] = zok ? B.x[((size_t)f * NYX + c) * g.N_z + zl] : 0.f;
+      delta[tid] = 0.f;
+    }
+    if (tid < 64) magb[tid] = 0.f;
+    __syncthreads();
+
+    // ---- surrogate update of every slice, red-black in z (or ascending)
+    const int n_steps = L.zorder ? g.N_z : 2;
+    for (int step = 0; step < n_steps; step++) {
+      const bool mine = L.zorder ? (zl == step) : ((zl & 1) == step);
+      if (tid < ZL && zok && mine) {
+        const int z = zl;
+        const float V = xcol[z];
+        float nb[27];
+        bool fl[27];
+        bool any_pos = false;
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+          for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+              const int l = ((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1);
+              const int z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+              const bool ok = (l != 13) && z2 >= 0 && z2 < g.N_z && y2 >= 0 && y2 < g.N_y &&
+                              x2 >= 0 && x2 < g.N_x;
+              float val = 0.f;
+              if (ok)
+                val = (dy == 0 && dx == 0)
+                          ? xcol[z2]
+                          : B.x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z2];
+              nb[l] = val;
+              fl[l] = ok;
+              any_pos |= ok && (val > 0.f);
+            }
+        float tn0 = 0.f, tn1 = 0.f;
+        const bool tf0 = f > 0, tf1 = f + 1 < g.N_time;
+        if (tf0) tn0 = B.x[((size_t)(f - 1) * NYX + c) * g.N_z + z];
+        if (tf1) tn1 = B.x[((size_t)(f + 1) * NYX + c) * g.N_z + z];
+        any_pos |= (tf0 && tn0 > 0.f) || (tf1 && tn1 > 0.f);
+        // zero skipping, F/XT_VoxUpdate.c:208-232
+        const bool skip = (V == 0.f) && (L.iter > 1) && !any_pos;
+        if (!skip) {
+          float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+          for (int l = 0; l < 27; l++) {
+            if (fl[l]) {
+              const float bq =
+                  g.filt[l] * xt_qggmrf_ratio(V - nb[l], g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+              s1 += bq * nb[l];
+              s2 += bq;
+            }
+          }
+          if (tf0) {
+            const float bq =
+                g.tfilt * xt_qggmrf_ratio(V - tn0, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+            s1 += bq * tn0;
+            s2 += bq;
+          }
+          if (tf1) {
+            const float bq =
+                g.tfilt * xt_qggmrf_ratio(V - tn1, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+            s1 += bq * tn1;
+            s2 += bq;
+          }
+          const float T1 = red1[z], T2 = red2[z];
+          const float u = (s1 + (T2 * V) - T1) / (s2 + T2);
+          float nv2 = V + g.alpha * (u - V);
+          if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
+          xcol[z] = nv2;
+          delta[z] = nv2 - V;
+          const int blk = g.N_z / g.nb;
+          int bidx = blk > 0 ? z / blk : 0;
+          if (bidx >= g.nb) bidx = g.nb - 1;
+          atomicAdd(&magb[bidx], fabsf(nv2 - V));
+          vox_mag += fabsf(nv2);
+          atomicAdd(&s_perf, 1);
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- write the line back and its update magnitude
+    if (tid < ZL && zok) B.x[((size_t)f * NYX + c) * g.N_z + zl] = xcol[zl];
+    if (tid < g.nb) B.mag_blk[((size_t)f * g.nb + tid) * NYX + c] = magb[tid];
+    if (tid == 0) {
+      float s = 0.f;
+      for (int b = 0; b < g.nb; b++) s += magb[b];
+      B.magsum[(size_t)f * NYX + c] = s;
+      n_upd += (double)s_perf;
+      n_foot += (double)s_perf * (double)s_cnt_col * (double)g.zo;
+    }
+
+    // ---- e -= A * delta over the same footprint
+    if (zok) {
+      const float dz = delta[zl];
+      if (dz != 0.f) {
+        for (int p = grp; p < nv; p += G) {
+          const int v = vlist[p];
+          const size_t base = ((size_t)v * g.N_r + a_first[p]) * g.N_t + zl;
+#pragma unroll
+          for (int b = 0; b < XT_MAXA; b++) {
+            const float a = a_vals[(size_t)p * XT_MAXA + b];
+            if (a > 0.f) {
+              const float pe = a * g.delta_t;
+              __hip_atomic_fetch_add(&B.e[base + (size_t)b * g.N_t], -(pe * dz), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_cnt_col = 0;
+      s_perf = 0;
+    }
+  }
+  // ---- per-workgroup statistics
+  if (tid < ZL && L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
+  if (tid == 0 && n_upd != 0.0) {
+    atomicAdd(&B.stats[1], n_upd);
+    atomicAdd(&B.stats[2], n_foot);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ring-offset update, F/XT_ICD_update.c:666-759.  The view sum of every
+// detector pixel is split over view chunks (enough workgroups to fill the
+// chip), then combined, then the zero-mean constraint gamma is formed, then
+// the correction is applied to d and to e.  Sums are FP64.
+__global__ void k_offset_partial(XtGeom g, const float *__restrict__ e, const float *__restrict__ w,
+                                 int chunk_views, double *__restrict__ part /*[nch][2][NRT]*/) {
+  const int NRT = g.N_r * g.N_t;
+  const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (rt >= NRT) return;
+  const int p0 = ch * chunk_views, p1 = min(g.N_p, p0 + chunk_views);
+  double num = 0.0, den = 0.0;
+  for (int p = p0; p < p1; p++) {
+    const size_t idx = (size_t)p * NRT + rt;
+    const float ev = e[idx], wv = w[idx];
+    if (ev * ev * wv < g.T * g.T) {
+      num += (double)ev * (double)wv;
+      den += (double)wv;
+    } else {
+      const double temp = (double)g.T * (double)g.delta * sqrt((double)wv);
+      const double sign = (ev > 0.f) - (ev < 0.f);
+      num += temp * sign;
+      den += temp / fabs((double)ev);
+    }
+  }
+  part[((size_t)ch * 2 + 0) * NRT + rt] = num;
+  part[((size_t)ch * 2 + 1) * NRT + rt] = den;
+}
+
+__global__ void k_offset_combine(XtGeom g, int nch, const double *__restrict__ part,
+                                 const float *__restrict__ d, double *__restrict__ numden /*[2][NRT]*/,
+                                 double *__restrict__ wg_sums /*[nblk][2]*/) {
+  __shared__ double sa[256], sb[256];
+  const int NRT = g.N_r * g.N_t;
+  const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (rt < NRT) {
+    double num = 0.0, den = 0.0;
+    for (int ch = 0; ch < nch; ch++) {
+      num += part[((size_t)ch * 2 + 0) * NRT + rt];
+      den += part[((size_t)ch * 2 + 1) * NRT + rt];
+    }
+    numden[rt] = num;
+    numden[NRT + rt] = den;
+    a = (double)d[rt] + num / den;
+    b = 1.0 / den;
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    wg_sums[blockIdx.x * 2 + 0] = sa[0];
+    wg_sums[blockIdx.x * 2 + 1] = sb[0];
+  }
+}
+
+__global__ void k_offset_gamma(int nblk, const double *__restrict__ wg_sums, int zero_mean,
+                               double *__restrict__ gamma) {
+  __shared__ double sa[256], sb[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    a += wg_sums[i * 2];
+    b += wg_sums[i * 2 + 1];
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *gamma = zero_mean ? sa[0] / sb[0] : 0.0;
+}
+
+__global__ void k_offset_apply(XtGeom g, int chunk_views, const double *__restrict__ numden,
+                               const double *__restrict__ gamma, float *__restrict__ d,
+                               float *__restrict__ e) {
+  const int NRT = g.N_r * g.N_t;
+  const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (rt >= NRT) return;
+  const float step = (float)((numden[rt] - *gamma) / numden[NRT + rt]);
+  if (ch == 0) d[rt] = d[rt] + step;
+  const int p0 = ch * chunk_views, p1 = min(g.N_p, p0 + chunk_views);
+  for (int p = p0; p < p1; p++) {
+    const size_t idx = (size_t)p * NRT + rt;
+    e[idx] -= step;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variance re-estimate, F/XT_ICD_update.c:633-664.
+__global__ void k_variance_partial(XtGeom g, const float *__restrict__ e, const float *__restrict__ w,
+                                   float var_old, size_t n, double *__restrict__ wg) {
+  __shared__ double sa[256];
+  double a = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float ev = e[i], wv = w[i];
+    const float wr = wv * var_old;  // Weight restored to 1/variance units
+    if (ev * ev * wv < g.T * g.T)
+      a += (double)(ev * ev * wr);
+    else
+      a += (double)fabsf(ev) * (double)g.delta * (double)g.T * sqrt((double)wr * (double)var_old);
+  }
+  sa[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sa[threadIdx.x] += sa[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wg[blockIdx.x] = sa[0];
+}
+
+__global__ void k_variance_apply(float *__restrict__ w, size_t n, float var_old, float var_new) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    w[i] = (w[i] * var_old) / var_new;
+}
+
+// ---------------------------------------------------------------------------
+// Generic FP64 block reduction of a float array (|x| optional) into wg[blockIdx].
+__global__ void k_sum_f32(const float *__restrict__ a, size_t n, int absval, double *__restrict__ wg) {
+  __shared__ double sa[256];
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    s += absval ? fabs((double)a[i]) : (double)a[i];
+  sa[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wg[blockIdx.x] = sa[0];
+}
+
+__global__ void k_sum_f64(const double *__restrict__ a, int n, double *__restrict__ out) {
+  __shared__ double sa[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+  sa[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sa[0];
+}
+
+// ---------------------------------------------------------------------------
+// MAP cost in FP64, F/XT_ICD_update.c:181-314: data term over the sinogram and
+// the prior over the 13 causal neighbour pairs plus the next frame.
+__global__ void k_cost_data(XtGeom g, const float *__restrict__ e, const float *__restrict__ w,
+                            size_t n, double *__restrict__ wg) {
+  __shared__ double sa[256];
+  double s = 0.0;
+  const double T = g.T, dl = g.delta;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double u = (double)e[i] * sqrt((double)w[i]);
+    const bool sel = e[i] * e[i] * w[i] < g.T * g.T;
+    s += sel ? u * u : 2.0 * dl * T * fabs(u) + T * T * (1.0 - 2.0 * dl);
+  }
+  sa[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wg[blockIdx.x] = sa[0] * 0.5;
+}
+
+__device__ __forceinline__ double xt_qggmrf_value(double delta, double p, double c, double sq,
+                                                  double sqp) {
+  const double ad = fabs(delta);
+  return (ad * ad / sq) / (c + pow(ad, 2.0 - p) / sqp);
+}
+
+__global__ void k_cost_prior(XtGeom g, const float *__restrict__ x, double *__restrict__ wg) {
+  __shared__ double sa[256];
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  const size_t n = (size_t)g.N_time * NYX * g.N_z;
+  double s = 0.0;
+  const double P = g.MRF_P;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % g.N_z);
+    const size_t col = (i / g.N_z) % NYX;
+    const int f = (int)(i / ((size_t)g.N_z * NYX));
+    const int yy = (int)(col / g.N_x), xx = (int)(col % g.N_x);
+    const double cv = x[i];
+    // causal neighbours: (dz,dy,dx) with dz=+1 (all 9), or dz=0 and (dy=+1 or dy=0,dx=+1)
+    for (int dz = 0; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if (dz == 0 && (dy < 0 || (dy == 0 && dx <= 0))) continue;
+          const int z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+          if (z2 >= g.N_z || y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x) continue;
+          const double nv = x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z2];
+          const int l = ((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1);
+          s += (double)g.filt[l] *
+               xt_qggmrf_value(cv - nv, P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+        }
+    if (f + 1 < g.N_time) {
+      const double nv = x[i + NYX * g.N_z];
+      s += (double)g.tfilt * xt_qggmrf_value(cv - nv, P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+    }
+  }
+  sa[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wg[blockIdx.x] = sa[0];
+}
+
+// ---------------------------------------------------------------------------
+// NHICD selection, F/XT_NHICD.c:128-216: 5x5 Hamming filter of each magnitude
+// map (float, reference order), then the order statistic of rank
+// N - n_select by radix select on the (non-negative) float bit patterns, then
+// the lines strictly above it.  One map per (blockIdx.y).
+__global__ void k_vsc_filter(int NY, int NX, const float *__restrict__ mag, float *__restrict__ filt) {
+#pragma clang fp contract(off)
+  const float Hm[5][5] = {{0.0064f, 0.0432f, 0.0800f, 0.0432f, 0.0064f},
+                          {0.0432f, 0.2916f, 0.5400f, 0.2916f, 0.0432f},
+                          {0.0800f, 0.5400f, 1.0000f, 0.5400f, 0.0800f},
+                          {0.0432f, 0.2916f, 0.5400f, 0.2916f, 0.0432f},
+                          {0.0064f, 0.0432f, 0.0800f, 0.0432f, 0.0064f}};
+  const size_t NYX = (size_t)NY * NX;
+  const int m = blockIdx.y;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int)NYX) return;
+  const int i = idx / NX, j = idx % NX;
+  const float *M = mag + (size_t)m * NYX;
+  float acc = 0;
+  for (int p = -2; p <= 2; p++)
+    for (int q = -2; q <= 2; q++)
+      if (i + p >= 0 && i + p < NY && j + q >= 0 && j + q < NX)
+        acc += Hm[p + 2][q + 2] * M[(size_t)(i + p) * NX + (j + q)];
+  filt[(size_t)m * NYX + idx] = acc;
+}
+
+__global__ void __launch_bounds__(1024) k_vsc_select(int NYX, const float *__restrict__ filt,
+                                                     const int *__restrict__ n_select,
+                                                     uint8_t *__restrict__ flag,
+                                                     int *__restrict__ n_out) {
+  __shared__ unsigned int hist[256];
+  __shared__ unsigned int s_prefix, s_mask;
+  __shared__ int s_target;
+  const int m = blockIdx.x;
+  const float *F = filt + (size_t)m * NYX;
+  const unsigned int *K = (const unsigned int *)F;
+  // 1-indexed rank from the smallest: i = N - n_select (F/XT_NHICD.c:185)
+  int target = NYX - n_select[m] - 1;  // 0-indexed
+  if (target < 0) target = 0;
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    s_mask = 0;
+    s_target = target;
+  }
+  __syncthreads();
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    const unsigned int pre = s_prefix, msk = s_mask;
+    for (int i = threadIdx.x; i < NYX; i += blockDim.x) {
+      const unsigned int k = K[i];
+      if ((k & msk) == pre) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = s_target;
+      unsigned int cum = 0;
+      int b = 0;
+      for (; b < 256; b++) {
+        if (cum + hist[b] > (unsigned int)t) break;
+        cum += hist[b];
+      }
+      if (b > 255) b = 255;
+      s_target = t - (int)cum;
+      s_prefix = pre | ((unsigned int)b << shift);
+      s_mask = msk | (255u << shift);
+    }
+    __syncthreads();
+  }
+  const float thresh = __uint_as_float(s_prefix);
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int local = 0;
+  for (int i = threadIdx.x; i < NYX; i += blockDim.x) {
+    const uint8_t s = F[i] > thresh ? 1 : 0;
+    flag[(size_t)m * NYX + i] = s;
+    local += s;
+  }
+  atomicAdd(&cnt, local);
+  __syncthreads();
+  if (threadIdx.x == 0) n_out[m] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Layout changes between the host's [N_time][N_z(+2)][N_y][N_x] object and the
+// device's [N_time][N_y][N_x][N_z].
+__global__ void k_object_to_device(XtGeom g, const float *__restrict__ host_layout /*[T][Z][Y][X]*/,
+                                   float *__restrict__ x) {
+  const size_t NYX = (size_t)g.N_y * g.N_x, n = (size_t)g.N_time * NYX * g.N_z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = i % NYX;
+    const int z = (int)((i / NYX) % g.N_z);
+    const size_t f = i / (NYX * g.N_z);
+    x[(f * NYX + col) * g.N_z + z] = host_layout[i];
+  }
+}
+
+__global__ void k_object_to_host(XtGeom g, const float *__restrict__ x,
+                                 float *__restrict__ host_layout) {
+  const size_t NYX = (size_t)g.N_y * g.N_x, n = (size_t)g.N_time * NYX * g.N_z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = i % NYX;
+    const int z = (int)((i / NYX) % g.N_z);
+    const size_t f = i / (NYX * g.N_z);
+    host_layout[i] = x[(f * NYX + col) * g.N_z + z];
+  }
+}
+
+// ProjSelect from the final e and w (F/XT_ICD_update.c:577).
+__global__ void k_select(XtGeom g, const float *__restrict__ e, const float *__restrict__ w, size_t n,
+                         uint8_t *__restrict__ sel) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    sel[i] = (fabs((double)e[i] * sqrt((double)w[i])) < (double)g.T) ? 1 : 0;
+}
+
+__global__ void k_zero_f64(double *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}

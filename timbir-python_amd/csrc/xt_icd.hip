@@ -1,0 +1,1207 @@
+// xt_icd.hip -- host side of libxt_hip.so: ICD_BackProject and the stepwise
+// and seam entry points of include/xt_icd.h.
+//
+// One process, one GPU, one stream.  Per ICD iteration the host enqueues, for
+// each of K = NHICD_Iterations sub-iterations: four colour launches of the
+// voxel-update kernel over 1/K of every tile's lines, the ring-offset update
+// after each half, the NHICD selection, and four colour launches over the
+// selected lines (from the second iteration on) -- the schedule of
+// Source_Code_4D_Fast/XT_ICD_update.c:767-926 with z-blocks replaced by whole
+// voxel lines and OpenMP threads replaced by super-voxel tiles.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xt_icd.h"
+#include "xt_device.h"
+
+#ifdef XT_REAL_IS_DOUBLE
+#error "libxt_hip is built with Real_t = float (the reference's production build)"
+#endif
+
+// ---------------------------------------------------------------------------
+// errors, options
+static thread_local char g_err[1024] = "";
+
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define XT_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (call);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return set_err(XT_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                     __FILE__, __LINE__);                                              \
+  } while (0)
+
+#define XT_TRY(expr)          \
+  do {                        \
+    int _rc = (expr);         \
+    if (_rc != XT_OK) return _rc; \
+  } while (0)
+
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
+
+extern "C" const char *xt_last_error(void) { return g_err; }
+extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
+
+extern "C" int xt_set_option(int option, double value) {
+  if (option < 1 || option > 8) return set_err(XT_E_ARG, "unknown option %d", option);
+  g_opt[option] = value;
+  return XT_OK;
+}
+extern "C" double xt_get_option(int option) {
+  return (option >= 1 && option <= 8) ? g_opt[option] : NAN;
+}
+
+static std::string run_path(const char *name) {
+  if (g_xt_run_dir.empty()) return std::string(name);
+  return g_xt_run_dir + "/" + name;
+}
+
+static void dbg(TomoInputs *T, const char *fmt, ...) {
+  if (!T || !T->debug_file_ptr || g_opt[XT_OPT_VERBOSE] == 0) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(T->debug_file_ptr, fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// pointer trees with the reference layout (F/allocate.c:91-188)
+extern "C" void *xt_multialloc(size_t s, int d, const int64_t *dims) {
+  if (d < 1) return nullptr;
+  if (d == 1) return calloc(dims[0] ? (size_t)dims[0] : 1, s);
+  size_t count = 1;
+  void **prev = nullptr;
+  void *top = nullptr;
+  for (int lvl = 0; lvl < d - 1; lvl++) {
+    size_t n = count * (size_t)dims[lvl];
+    void **arr = (void **)calloc(n ? n : 1, sizeof(void *));
+    if (!arr) return nullptr;
+    if (lvl == 0)
+      top = arr;
+    else
+      for (size_t j = 0; j < count; j++) prev[j] = (void *)(arr + j * (size_t)dims[lvl]);
+    prev = arr;
+    count = n;
+  }
+  size_t row = (size_t)dims[d - 1] * s;
+  char *data = (char *)calloc(count * (size_t)dims[d - 1] ? count * (size_t)dims[d - 1] : 1, s);
+  if (!data) return nullptr;
+  for (size_t j = 0; j < count; j++) prev[j] = data + j * row;
+  return top;
+}
+
+extern "C" void xt_multifree(void *r, int d) {
+  void **p = (void **)r;
+  for (int i = 0; i < d && p != nullptr; i++) {
+    void *next = *p;
+    free(p);
+    p = (void **)next;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// geometry from the engine structs
+static int zl_for(int nz) { return nz <= 8 ? 8 : nz <= 16 ? 16 : nz <= 32 ? 32 : 64; }
+
+static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs *T, XtGeom *g) {
+  memset(g, 0, sizeof *g);
+  if (S->N_p <= 0 || S->N_r <= 0 || S->N_t <= 0 || O->N_x <= 0 || O->N_y <= 0 || O->N_z <= 0 ||
+      O->N_time <= 0)
+    return set_err(XT_E_ARG, "empty problem (N_p=%d N_r=%d N_t=%d N_x=%d N_y=%d N_z=%d N_time=%d)",
+                   S->N_p, S->N_r, S->N_t, O->N_x, O->N_y, O->N_z, O->N_time);
+  if (S->z_overlap_num != 1 || S->N_t != O->N_z)
+    return set_err(XT_E_ARG, "z_overlap_num=%d (N_t=%d, N_z=%d): only delta_z = 1 is supported",
+                   S->z_overlap_num, S->N_t, O->N_z);
+  if (O->N_z > 64) return set_err(XT_E_ARG, "N_z=%d > 64 slices per GPU", O->N_z);
+  if (2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA)
+    return set_err(XT_E_ARG, "delta_xy/delta_r = %g too large for one A column",
+                   (double)(O->delta_xy / S->delta_r));
+  if (T && T->node_num > 1)
+    return set_err(XT_E_ARG, "node_num=%d: this engine runs one slab on one GPU", T->node_num);
+  g->N_p = S->N_p;
+  g->N_r = S->N_r;
+  g->N_t = S->N_t;
+  g->N_x = O->N_x;
+  g->N_y = O->N_y;
+  g->N_z = O->N_z;
+  g->N_time = O->N_time;
+  g->zo = S->z_overlap_num;
+  g->nb = (T && T->num_z_blocks > 0) ? T->num_z_blocks : 1;
+  if (g->nb > 64) g->nb = 64;
+  g->delta_r = S->delta_r;
+  g->delta_t = S->delta_t;
+  g->R0 = S->R0;
+  g->RMax = S->RMax;
+  g->OffsetR = S->OffsetR;
+  g->x0 = O->x0;
+  g->y0 = O->y0;
+  g->delta_xy = O->delta_xy;
+  g->MRF_P = O->MRF_P;
+  g->C_S = O->C_S;
+  g->C_T = O->C_T;
+  if (T) {
+    g->Sigma_S_Q = T->Sigma_S_Q;
+    g->Sigma_S_Q_P = T->Sigma_S_Q_P;
+    g->Sigma_T_Q = T->Sigma_T_Q;
+    g->Sigma_T_Q_P = T->Sigma_T_Q_P;
+    memcpy(g->filt, &T->Spatial_Filter[0][0][0], sizeof g->filt);
+    g->tfilt = T->Time_Filter[0];
+    g->alpha = T->alpha;
+    g->T = T->ErrorSinoThresh;
+    g->delta = T->ErrorSinoDelta;
+  }
+  g->positivity = g_opt[XT_OPT_POSITIVITY] != 0;
+  int mv = 0;
+  for (int i = 0; i < O->N_time; i++) mv = O->ProjNum[i] > mv ? O->ProjNum[i] : mv;
+  g->max_views = mv > 0 ? mv : 1;
+  return XT_OK;
+}
+
+static size_t icd_lds_bytes(const XtGeom &g, int ZL) {
+  return (size_t)g.max_views * (XT_MAXA * sizeof(float) + sizeof(int)) + 2 * 256 * sizeof(float) +
+         2 * ZL * sizeof(float) + 64 * sizeof(float) + 256 * sizeof(int);
+}
+
+template <typename T>
+static int dev_alloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return set_err(XT_E_HIP, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return XT_OK;
+}
+
+static int grid_1d(size_t n, int bs = 256, int cap = 8192) {
+  size_t b = (n + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > (size_t)cap) b = cap;
+  return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// stepwise state
+struct xt_icd_state {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  XtGeom g;
+  int ZL = 32;
+  int K = 10, NumIter = 0, update_offset = 0, zero_mean = 0, update_var = 0, no_nhicd = 0;
+  float var_est = 1.f, stop_threshold = 0.f;
+  int iter = 0, stopped = 0;
+  int TS = 8, n_masked = 0;
+  size_t nsino = 0, nobj = 0, NYX = 0;
+  TomoInputs *T = nullptr;  // for debug lines only
+  // device buffers
+  float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *x = nullptr, *Hr = nullptr;
+  float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr;
+  int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
+  int *tiles_by_color = nullptr, *n_tiles = nullptr, *tile_start = nullptr, *tile_cols = nullptr;
+  int max_tiles = 0, n_tiles_tot = 0;
+  uint8_t *nhicd_flag = nullptr;
+  float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr;
+  int *nsel = nullptr, *nsel_out = nullptr;
+  double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
+  double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
+  int off_nch = 1, off_chunk = 1, off_nblk = 1;
+  // statistics
+  double total_updates = 0, total_foot = 0, icd_ms = 0, last_cost = NAN, stop_pct = NAN;
+  int64_t launches = 0;
+  int timing = 0;
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;
+};
+
+static void free_state(xt_icd_state *s) {
+  if (!s) return;
+  void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
+                  s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
+                  s->tiles_by_color, s->n_tiles, s->tile_start, s->tile_cols, s->nhicd_flag,
+                  s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
+                  s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+static int launch_detector_response(xt_icd_state *s, float beam_width) {
+  hipLaunchKernelGGL(k_detector_response, dim3(s->g.N_p), dim3(256), 0, s->st, s->g, s->viewang,
+                     beam_width, s->Hr);
+  XT_HIP(hipGetLastError());
+  return XT_OK;
+}
+
+template <int ZL>
+static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const float *Hr,
+                      const float *c, const float *s, const int *vf, int mode, double *out64,
+                      const float *y, const float *d, float *e) {
+  constexpr int RB = 256 / ZL;
+  dim3 grid((g.N_r + RB - 1) / RB, g.N_p);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL>), grid, dim3(256), 0, st, g, x, Hr, c, s,
+                     vf, mode, out64, y, d, e);
+}
+
+static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *x, const float *Hr,
+                          const float *c, const float *s, const int *vf, int mode, double *out64,
+                          const float *y, const float *d, float *e) {
+  switch (ZL) {
+    case 8: fp_launch<8>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
+    case 16: fp_launch<16>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
+    case 32: fp_launch<32>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
+    default: fp_launch<64>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
+  }
+  XT_HIP(hipGetLastError());
+  return XT_OK;
+}
+
+template <int ZL>
+static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
+                       int gx, size_t lds) {
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL>), dim3(gx, g.N_time), dim3(256), lds, st, g, B,
+                     L);
+}
+
+static XtIcdBuffers buffers_of(xt_icd_state *s) {
+  XtIcdBuffers B;
+  B.e = s->e;
+  B.w = s->w;
+  B.x = s->x;
+  B.Hr = s->Hr;
+  B.cosv = s->cosv;
+  B.sinv = s->sinv;
+  B.views = s->views;
+  B.proj_num = s->proj_num;
+  B.tiles_by_color = s->tiles_by_color;
+  B.n_tiles = s->n_tiles;
+  B.max_tiles = s->max_tiles;
+  B.tile_start = s->tile_start;
+  B.tile_cols = s->tile_cols;
+  B.nhicd_flag = s->nhicd_flag;
+  B.mag_blk = s->mag_blk;
+  B.magsum = s->magsum;
+  B.stats = s->stats;
+  return B;
+}
+
+static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L) {
+  XtIcdBuffers B = buffers_of(s);
+  size_t lds = icd_lds_bytes(s->g, s->ZL);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing) {
+    if (s->ev_used + 2 > (int)s->ev.size()) {
+      for (int i = 0; i < 64; i++) {
+        hipEvent_t e;
+        XT_HIP(hipEventCreate(&e));
+        s->ev.push_back(e);
+      }
+    }
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    XT_HIP(hipEventRecord(e0, s->st));
+  }
+  switch (s->ZL) {
+    case 8: icd_launch<8>(s->st, s->g, B, L, s->max_tiles, lds); break;
+    case 16: icd_launch<16>(s->st, s->g, B, L, s->max_tiles, lds); break;
+    case 32: icd_launch<32>(s->st, s->g, B, L, s->max_tiles, lds); break;
+    default: icd_launch<64>(s->st, s->g, B, L, s->max_tiles, lds); break;
+  }
+  XT_HIP(hipGetLastError());
+  if (s->timing) XT_HIP(hipEventRecord(e1, s->st));
+  s->launches++;
+  return XT_OK;
+}
+
+static int offset_update(xt_icd_state *s) {
+  const int NRT = s->g.N_r * s->g.N_t;
+  dim3 g2((NRT + 255) / 256, s->off_nch);
+  hipLaunchKernelGGL(k_offset_partial, g2, dim3(256), 0, s->st, s->g, s->e, s->w, s->off_chunk,
+                     s->off_part);
+  hipLaunchKernelGGL(k_offset_combine, dim3(s->off_nblk), dim3(256), 0, s->st, s->g, s->off_nch,
+                     s->off_part, s->d, s->off_numden, s->off_wg);
+  hipLaunchKernelGGL(k_offset_gamma, dim3(1), dim3(256), 0, s->st, s->off_nblk, s->off_wg,
+                     s->zero_mean, s->scal);
+  hipLaunchKernelGGL(k_offset_apply, g2, dim3(256), 0, s->st, s->g, s->off_chunk, s->off_numden,
+                     s->scal, s->d, s->e);
+  XT_HIP(hipGetLastError());
+  return XT_OK;
+}
+
+static int variance_update(xt_icd_state *s) {
+  int nb = grid_1d(s->nsino, 256, 4096);
+  hipLaunchKernelGGL(k_variance_partial, dim3(nb), dim3(256), 0, s->st, s->g, s->e, s->w, s->var_est,
+                     s->nsino, s->red_wg);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 1);
+  XT_HIP(hipGetLastError());
+  double sum = 0;
+  XT_HIP(hipMemcpyAsync(&sum, s->scal + 1, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipStreamSynchronize(s->st));
+  float var_old = s->var_est;
+  float var_new = (float)(sum / ((double)s->g.N_p * s->g.N_r * s->g.N_t));
+  hipLaunchKernelGGL(k_variance_apply, dim3(nb), dim3(256), 0, s->st, s->w, s->nsino, var_old,
+                     var_new);
+  XT_HIP(hipGetLastError());
+  s->var_est = var_new;
+  return XT_OK;
+}
+
+static int vsc_select(xt_icd_state *s) {
+  dim3 gf((unsigned)((s->NYX + 255) / 256), s->g.N_time);
+  hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->magsum, s->filt);
+  hipLaunchKernelGGL(k_vsc_select, dim3(s->g.N_time), dim3(1024), 0, s->st, (int)s->NYX, s->filt,
+                     s->nsel, s->nhicd_flag, s->nsel_out);
+  XT_HIP(hipGetLastError());
+  return XT_OK;
+}
+
+static int build_tiles(xt_icd_state *s, const std::vector<int> &masked_cols) {
+  const int NY = s->g.N_y, NX = s->g.N_x;
+  int TS = (int)g_opt[XT_OPT_TILE];
+  if (TS <= 0) {
+    // concurrent tiles per colour: at most 1024 and at most 1/64 of the lines
+    int ncol = NY * NX;
+    int target = ncol / 64;
+    if (target > 1024) target = 1024;
+    if (target < 16) target = 16;
+    TS = 1;
+    while (TS < 16) {
+      int per_color = ((NY + TS - 1) / TS) * ((NX + TS - 1) / TS) / 4;
+      if (per_color <= target) break;
+      TS *= 2;
+    }
+  }
+  if (TS < 1 || TS > 16) return set_err(XT_E_ARG, "tile side %d outside 1..16", TS);
+  s->TS = TS;
+  const int TY = (NY + TS - 1) / TS, TX = (NX + TS - 1) / TS;
+  std::vector<std::vector<int>> cols(TY * TX);
+  std::vector<uint8_t> in(NY * NX, 0);
+  for (int c : masked_cols) in[c] = 1;
+  for (int yy = 0; yy < NY; yy++)
+    for (int xx = 0; xx < NX; xx++)
+      if (in[yy * NX + xx]) cols[(yy / TS) * TX + (xx / TS)].push_back(yy * NX + xx);
+  std::vector<int> start(1, 0), flat, bycolor[4];
+  int nt = 0;
+  std::vector<int> tile_id(TY * TX, -1);
+  for (int ty = 0; ty < TY; ty++)
+    for (int tx = 0; tx < TX; tx++) {
+      auto &v = cols[ty * TX + tx];
+      if (v.empty()) continue;
+      flat.insert(flat.end(), v.begin(), v.end());
+      start.push_back((int)flat.size());
+      bycolor[(ty & 1) * 2 + (tx & 1)].push_back(nt);
+      nt++;
+    }
+  int mt = 1;
+  for (int c = 0; c < 4; c++) mt = (int)bycolor[c].size() > mt ? (int)bycolor[c].size() : mt;
+  std::vector<int> tbc(4 * mt, 0), ntc(4);
+  for (int c = 0; c < 4; c++) {
+    ntc[c] = (int)bycolor[c].size();
+    for (size_t i = 0; i < bycolor[c].size(); i++) tbc[c * mt + i] = bycolor[c][i];
+  }
+  s->max_tiles = mt;
+  s->n_tiles_tot = nt;
+  XT_TRY(dev_alloc(&s->tiles_by_color, tbc.size()));
+  XT_TRY(dev_alloc(&s->n_tiles, 4));
+  XT_TRY(dev_alloc(&s->tile_start, start.size()));
+  XT_TRY(dev_alloc(&s->tile_cols, flat.size()));
+  XT_HIP(hipMemcpy(s->tiles_by_color, tbc.data(), tbc.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->n_tiles, ntc.data(), 4 * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->tile_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!flat.empty())
+    XT_HIP(hipMemcpy(s->tile_cols, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+  return XT_OK;
+}
+
+// the detector-table kernel reduces angles by repeated subtraction of pi/2
+static int check_angles(const Sinogram *S) {
+  for (int i = 0; i < S->N_p; i++)
+    if (!std::isfinite(S->ViewPtr[i]) || fabsf(S->ViewPtr[i]) > 1.0e5f)
+      return set_err(XT_E_ARG, "view %d angle %g rad outside +-1e5", i, (double)S->ViewPtr[i]);
+  return XT_OK;
+}
+
+static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject *O) {
+  const XtGeom &g = s->g;
+  XT_TRY(check_angles(S));
+  std::vector<int> flat((size_t)g.N_time * g.max_views, 0), num(g.N_time), vf(g.N_p, -1);
+  for (int f = 0; f < g.N_time; f++) {
+    num[f] = O->ProjNum[f];
+    for (int p = 0; p < O->ProjNum[f]; p++) {
+      int v = O->ProjIdxPtr[f][p];
+      if (v < 0 || v >= g.N_p) return set_err(XT_E_ARG, "view index %d out of range", v);
+      flat[(size_t)f * g.max_views + p] = v;
+      vf[v] = f;
+    }
+  }
+  XT_TRY(dev_alloc(&s->views, flat.size()));
+  XT_TRY(dev_alloc(&s->proj_num, g.N_time));
+  XT_TRY(dev_alloc(&s->view_frame, g.N_p));
+  XT_TRY(dev_alloc(&s->cosv, g.N_p));
+  XT_TRY(dev_alloc(&s->sinv, g.N_p));
+  XT_TRY(dev_alloc(&s->viewang, g.N_p));
+  XT_HIP(hipMemcpy(s->views, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->proj_num, num.data(), num.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->view_frame, vf.data(), vf.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->cosv, S->cosine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->sinv, S->sine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, g.N_p * sizeof(float), hipMemcpyHostToDevice));
+  return XT_OK;
+}
+
+static int upload_object(xt_icd_state *s, const ScannedObject *O, float *x_dev) {
+  const XtGeom &g = s->g;
+  size_t per = (size_t)g.N_z * g.N_y * g.N_x;
+  float *tmp = nullptr;
+  XT_TRY(dev_alloc(&tmp, per * g.N_time));
+  for (int f = 0; f < g.N_time; f++) {
+    hipError_t e = hipMemcpy(tmp + per * f, &O->Object[f][1][0][0], per * sizeof(float),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(tmp);
+      return set_err(XT_E_HIP, "object upload: %s", hipGetErrorString(e));
+    }
+  }
+  hipLaunchKernelGGL(k_object_to_device, dim3(grid_1d(per * g.N_time)), dim3(256), 0, s->st, g, tmp,
+                     x_dev);
+  hipError_t e1 = hipStreamSynchronize(s->st);
+  (void)hipFree(tmp);
+  if (e1 != hipSuccess) return set_err(XT_E_HIP, "object layout: %s", hipGetErrorString(e1));
+  return XT_OK;
+}
+
+static int download_object(xt_icd_state *s, ScannedObject *O) {
+  const XtGeom &g = s->g;
+  size_t per = (size_t)g.N_z * g.N_y * g.N_x;
+  float *tmp = nullptr;
+  XT_TRY(dev_alloc(&tmp, per * g.N_time));
+  hipLaunchKernelGGL(k_object_to_host, dim3(grid_1d(per * g.N_time)), dim3(256), 0, s->st, g, s->x,
+                     tmp);
+  hipError_t e = hipStreamSynchronize(s->st);
+  for (int f = 0; f < g.N_time && e == hipSuccess; f++)
+    e = hipMemcpy(&O->Object[f][1][0][0], tmp + per * f, per * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "object download: %s", hipGetErrorString(e));
+  return XT_OK;
+}
+
+static int init_device(int *dev) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return set_err(XT_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  *dev = (int)g_opt[XT_OPT_DEVICE];
+  if (*dev < 0 || *dev >= n) return set_err(XT_E_ARG, "device %d of %d", *dev, n);
+  XT_HIP(hipSetDevice(*dev));
+  return XT_OK;
+}
+
+static int icd_resync(xt_icd_state *s) {
+  return launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 1,
+                        nullptr, s->y, s->d, s->e);
+}
+
+static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **out) {
+  *out = nullptr;
+  xt_icd_state *s = new xt_icd_state();
+  auto fail = [&](int rc) {
+    free_state(s);
+    return rc;
+  };
+  int rc = init_device(&s->dev);
+  if (rc) return fail(rc);
+  rc = make_geom(S, O, T, &s->g);
+  if (rc) return fail(rc);
+  if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_err(XT_E_HIP, "hipStreamCreate"));
+  XtGeom &g = s->g;
+  s->ZL = zl_for(g.N_z);
+  s->T = T;
+  s->K = O->NHICD_Iterations > 0 ? O->NHICD_Iterations : 1;
+  s->NumIter = T->NumIter;
+  s->update_offset = T->updateProjOffset;
+  s->zero_mean = T->EnforceZeroMeanOffset;
+  s->update_var = T->updateVar;
+  s->no_nhicd = T->no_NHICD;
+  s->var_est = T->var_est;
+  s->stop_threshold = T->StopThreshold;
+  if (icd_lds_bytes(g, s->ZL) > 64 * 1024)
+    return fail(set_err(XT_E_ARG, "%d views in one frame exceed the LDS footprint cache",
+                        g.max_views));
+  s->nsino = (size_t)g.N_p * g.N_r * g.N_t;
+  s->NYX = (size_t)g.N_y * g.N_x;
+  s->nobj = (size_t)g.N_time * s->NYX * g.N_z;
+  // masked lines, F/XT_ICD_update.c:958-974
+  std::vector<int> masked;
+  for (int j = 0; j < g.N_y; j++)
+    for (int k = 0; k < g.N_x; k++) {
+      float xc = (float)((double)O->x0 + ((double)(float)k + 0.5) * (double)O->delta_xy);
+      float yc = (float)((double)O->y0 + ((double)(float)j + 0.5) * (double)O->delta_xy);
+      if (xc * xc + yc * yc < T->radius_obj * T->radius_obj) masked.push_back(j * g.N_x + k);
+    }
+  s->n_masked = (int)masked.size();
+  if ((rc = build_tiles(s, masked))) return fail(rc);
+  if ((rc = upload_views(s, S, O))) return fail(rc);
+  if ((rc = dev_alloc(&s->e, s->nsino)) || (rc = dev_alloc(&s->w, s->nsino)) ||
+      (rc = dev_alloc(&s->y, s->nsino)) || (rc = dev_alloc(&s->d, (size_t)g.N_r * g.N_t)) ||
+      (rc = dev_alloc(&s->x, s->nobj)) || (rc = dev_alloc(&s->Hr, (size_t)g.N_p * XT_HR_STRIDE)) ||
+      (rc = dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX)) ||
+      (rc = dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
+      (rc = dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX)) ||
+      (rc = dev_alloc(&s->filt, (size_t)g.N_time * s->NYX)) ||
+      (rc = dev_alloc(&s->nsel, g.N_time)) || (rc = dev_alloc(&s->nsel_out, g.N_time)) ||
+      (rc = dev_alloc(&s->stats, 8)) || (rc = dev_alloc(&s->red_wg, 8192)) ||
+      (rc = dev_alloc(&s->scal, 8)))
+    return fail(rc);
+  // ring-offset scratch: view chunks so that N_r*N_t*chunks >= ~256k threads
+  {
+    const int NRT = g.N_r * g.N_t;
+    int nch = (262144 + NRT - 1) / NRT;
+    if (nch > g.N_p) nch = g.N_p;
+    if (nch < 1) nch = 1;
+    s->off_chunk = (g.N_p + nch - 1) / nch;
+    s->off_nch = (g.N_p + s->off_chunk - 1) / s->off_chunk;
+    s->off_nblk = (NRT + 255) / 256;
+    if ((rc = dev_alloc(&s->off_part, (size_t)s->off_nch * 2 * NRT)) ||
+        (rc = dev_alloc(&s->off_numden, (size_t)2 * NRT)) ||
+        (rc = dev_alloc(&s->off_wg, (size_t)2 * s->off_nblk)))
+      return fail(rc);
+  }
+  if (hipMemcpy(s->y, &S->Projection[0][0][0], s->nsino * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(s->w, &T->Weight[0][0][0], s->nsino * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t * sizeof(float),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(XT_E_HIP, "sinogram upload failed"));
+  if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
+      hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)) != hipSuccess ||
+      hipMemset(s->nhicd_flag, 0, (size_t)g.N_time * s->NYX) != hipSuccess)
+    return fail(set_err(XT_E_HIP, "scratch clear failed"));
+  {
+    // NHICD lines per frame: floor(UpdateSelectNum / NHICD_Iterations), F/XT_NHICD.c:182
+    std::vector<int> ns(g.N_time, s->n_masked / s->K);
+    if (hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(XT_E_HIP, "nsel upload"));
+  }
+  if ((rc = upload_object(s, O, s->x))) return fail(rc);
+  if ((rc = launch_detector_response(s, O->BeamWidth))) return fail(rc);
+  if ((rc = icd_resync(s))) return fail(rc);
+  if (hipStreamSynchronize(s->st) != hipSuccess)
+    return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
+  *out = s;
+  return XT_OK;
+}
+
+extern "C" int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state) {
+  if (!S || !O || !T || !state) return set_err(XT_E_ARG, "null argument");
+  return create_impl(S, O, T, state);
+}
+
+extern "C" int xt_icd_set_timing(xt_icd_state *s, int32_t on) {
+  if (!s) return set_err(XT_E_ARG, "null state");
+  s->timing = on;
+  return XT_OK;
+}
+
+static int collect_timing(xt_icd_state *s) {
+  if (!s->timing) return XT_OK;
+  for (int i = 0; i + 1 < s->ev_used; i += 2) {
+    float ms = 0;
+    XT_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+    s->icd_ms += ms;
+  }
+  s->ev_used = 0;
+  return XT_OK;
+}
+
+extern "C" int xt_icd_cost(xt_icd_state *s, double *cost) {
+  if (!s || !cost) return set_err(XT_E_ARG, "null argument");
+  int nb = grid_1d(s->nsino, 256, 4096);
+  hipLaunchKernelGGL(k_cost_data, dim3(nb), dim3(256), 0, s->st, s->g, s->e, s->w, s->nsino,
+                     s->red_wg);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 2);
+  int nbp = grid_1d(s->nobj, 256, 4096);
+  hipLaunchKernelGGL(k_cost_prior, dim3(nbp), dim3(256), 0, s->st, s->g, s->x, s->red_wg);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nbp, s->scal + 3);
+  XT_HIP(hipGetLastError());
+  double v[2];
+  XT_HIP(hipMemcpyAsync(v, s->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipStreamSynchronize(s->st));
+  const double n = (double)s->g.N_p * s->g.N_r * s->g.N_t;
+  *cost = v[0] + n * log((double)s->var_est) / 2 + v[1];
+  return XT_OK;
+}
+
+static int iterate_one(xt_icd_state *s) {
+  const int Iter = ++s->iter;
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s->st, s->stats, 8);
+  XtIcdLaunch L;
+  L.zorder = 0;
+  L.K = s->K;
+  L.iter = Iter;
+  L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
+  const bool nhicd = Iter > 1 && !s->no_nhicd;
+  for (int j = 0; j < s->K; j++) {
+    L.sub_iter = j;
+    L.mode = 0;
+    L.count_stats = 1;
+    for (int ph = 0; ph < 4; ph++) {
+      L.phase = ph;
+      XT_TRY(launch_icd(s, L));
+      if (s->update_offset > 1 && (ph & 1)) XT_TRY(offset_update(s));
+    }
+    if (nhicd) {
+      XT_TRY(vsc_select(s));
+      L.mode = 1;
+      L.count_stats = 0;
+      for (int ph = 0; ph < 4; ph++) {
+        L.phase = ph;
+        XT_TRY(launch_icd(s, L));
+        if (s->update_offset > 1 && (ph & 1)) XT_TRY(offset_update(s));
+      }
+    }
+  }
+  if (s->update_var == 1) XT_TRY(variance_update(s));
+  // stop statistic, F/XT_ICD_update.c:887-925
+  int nb = grid_1d((size_t)s->g.N_time * s->NYX, 256, 4096);
+  hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, s->st, s->magsum,
+                     (size_t)s->g.N_time * s->NYX, 0, s->red_wg);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 4);
+  XT_HIP(hipGetLastError());
+  double st[3], avg = 0;
+  XT_HIP(hipMemcpyAsync(st, s->stats, 3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipMemcpyAsync(&avg, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_TRY(collect_timing(s));
+  s->total_updates += st[1];
+  s->total_foot += st[2];
+  s->stop_pct = 100.0 * avg / st[0];
+  s->stopped = (s->stop_pct < (double)s->stop_threshold) ? 1 : 0;
+  int every = (int)g_opt[XT_OPT_RESYNC_EVERY];
+  if (every > 0 && Iter % every == 0) XT_TRY(icd_resync(s));
+  if (g_opt[XT_OPT_COST] != 0) XT_TRY(xt_icd_cost(s, &s->last_cost));
+  dbg(s->T, "ICD iteration %d: updates %.0f, average update %.6f%%, cost %.9g\n", Iter, st[1],
+      s->stop_pct, s->last_cost);
+  return XT_OK;
+}
+
+extern "C" int xt_icd_iterate(xt_icd_state *s, int32_t n_iter, int32_t *stop) {
+  if (!s) return set_err(XT_E_ARG, "null state");
+  for (int i = 0; i < n_iter; i++) XT_TRY(iterate_one(s));
+  if (stop) *stop = s->stopped;
+  return XT_OK;
+}
+
+extern "C" int xt_icd_get_stats(xt_icd_state *s, xt_icd_stats *out) {
+  if (!s || !out) return set_err(XT_E_ARG, "null argument");
+  out->iterations = s->iter;
+  out->voxel_updates = s->total_updates;
+  out->footprint_entries = s->total_foot;
+  out->icd_kernel_ms = s->icd_ms;
+  out->icd_launches = s->launches;
+  out->last_cost = s->last_cost;
+  out->stop_percentage = s->stop_pct;
+  out->stopped = s->stopped;
+  return XT_OK;
+}
+
+extern "C" int xt_icd_resync(xt_icd_state *s) {
+  if (!s) return set_err(XT_E_ARG, "null state");
+  XT_TRY(icd_resync(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
+extern "C" int xt_icd_synchronize(xt_icd_state *s) {
+  if (!s) return set_err(XT_E_ARG, "null state");
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
+extern "C" int xt_icd_download(xt_icd_state *s, Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  if (!s || !S || !O || !T) return set_err(XT_E_ARG, "null argument");
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_TRY(download_object(s, O));
+  XT_HIP(hipMemcpy(&S->ProjOffset[0][0], s->d, (size_t)s->g.N_r * s->g.N_t * sizeof(float),
+                   hipMemcpyDeviceToHost));
+  if (s->update_var == 1) {
+    XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+    T->var_est = s->var_est;
+  }
+  uint8_t *sel = nullptr;
+  XT_TRY(dev_alloc(&sel, s->nsino));
+  hipLaunchKernelGGL(k_select, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->e, s->w,
+                     s->nsino, sel);
+  hipError_t e = hipStreamSynchronize(s->st);
+  if (e == hipSuccess)
+    e = hipMemcpy(&S->ProjSelect[0][0][0], sel, s->nsino, hipMemcpyDeviceToHost);
+  (void)hipFree(sel);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "ProjSelect download: %s", hipGetErrorString(e));
+  return XT_OK;
+}
+
+extern "C" void xt_icd_destroy(xt_icd_state *s) { free_state(s); }
+
+// ---------------------------------------------------------------------------
+// the drop-in boundary
+static int write_run_bin(const char *name, const void *data, size_t count) {
+  std::string p = run_path(name);
+  FILE *fp = fopen(p.c_str(), "wb");
+  if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
+  size_t w = fwrite(data, sizeof(float), count, fp);
+  fclose(fp);
+  if (w != count) return set_err(XT_E_IO, "short write to %s", p.c_str());
+  return XT_OK;
+}
+
+static int read_run_bin(const char *name, void *data, size_t count) {
+  std::string p = run_path(name);
+  FILE *fp = fopen(p.c_str(), "rb");
+  if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
+  size_t r = fread(data, sizeof(float), count, fp);
+  fclose(fp);
+  if (r != count) return set_err(XT_E_IO, "%s holds %zu of %zu values", p.c_str(), r, count);
+  return XT_OK;
+}
+
+extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  if (!S || !O || !T) return set_err(XT_E_ARG, "null argument");
+  char name[256];
+  // initObject, F/XT_ICD_update.c:443-536 (initICD 0 and 1; 2 and 3 are the
+  // multi-resolution upsampling modes, not part of this path)
+  for (int i = 0; i < O->N_time; i++)
+    memset(&O->Object[i][0][0][0], 0, (size_t)(O->N_z + 2) * O->N_y * O->N_x * sizeof(float));
+  if (T->initICD == 1) {
+    for (int i = 0; i < O->N_time; i++) {
+      snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
+      XT_TRY(read_run_bin(name, &O->Object[i][1][0][0], (size_t)O->N_z * O->N_y * O->N_x));
+    }
+  } else if (T->initICD != 0) {
+    return set_err(XT_E_ARG, "initICD=%d (multi-resolution upsampling) is not supported",
+                   (int)T->initICD);
+  }
+  xt_icd_state *s = nullptr;
+  XT_TRY(create_impl(S, O, T, &s));
+  dbg(T, "ICD_BackProject: %s, tiles %dx%d (%d), z lanes %d\n", xt_version(), s->TS, s->TS,
+      s->n_tiles_tot, s->ZL);
+  // the error sinogram now lives on the GPU; Projection is released as the
+  // reference does (F/XT_ICD_update.c:999)
+  xt_multifree(S->Projection, 3);
+  S->Projection = nullptr;
+  int rc = XT_OK;
+  if (g_opt[XT_OPT_COST] != 0) {
+    rc = xt_icd_cost(s, &s->last_cost);
+    dbg(T, "ICD_BackProject: cost before ICD %.9g\n", s->last_cost);
+  }
+  for (int Iter = 1; rc == XT_OK && Iter <= T->NumIter; Iter++) {
+    rc = iterate_one(s);
+    if (rc == XT_OK && s->stopped && Iter > 1) {
+      dbg(T, "ICD_BackProject: Convergence criteria is met\n");
+      break;
+    }
+  }
+  if (rc == XT_OK) rc = xt_icd_download(s, S, O, T);
+  if (rc == XT_OK) {
+    snprintf(name, sizeof name, "variance_estimate_n%d.bin", T->node_rank);
+    rc = write_run_bin(name, &T->var_est, 1);
+  }
+  if (rc == XT_OK) {
+    size_t n = (size_t)s->g.N_time * s->g.nb * s->NYX;
+    std::vector<float> mag(n);
+    if (hipMemcpy(mag.data(), s->mag_blk, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = set_err(XT_E_HIP, "magnitude map download");
+    snprintf(name, sizeof name, "mag_update_map_n%d.bin", T->node_rank);
+    if (rc == XT_OK) rc = write_run_bin(name, mag.data(), n);
+  }
+  dbg(T, "ICD_BackProject: Finished (%d iterations)\n", s->iter);
+  free_state(s);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// test seams
+struct SeamCtx {
+  xt_icd_state *s = nullptr;
+  ~SeamCtx() { free_state(s); }
+};
+
+static int seam_state(SeamCtx &c, Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  c.s = new xt_icd_state();
+  XT_TRY(init_device(&c.s->dev));
+  XT_TRY(make_geom(S, O, T, &c.s->g));
+  XT_HIP(hipStreamCreate(&c.s->st));
+  c.s->ZL = zl_for(c.s->g.N_z);
+  c.s->nsino = (size_t)S->N_p * S->N_r * S->N_t;
+  c.s->NYX = (size_t)O->N_y * O->N_x;
+  c.s->nobj = (size_t)O->N_time * c.s->NYX * O->N_z;
+  c.s->T = T;
+  if (T) {
+    c.s->var_est = T->var_est;
+    c.s->zero_mean = T->EnforceZeroMeanOffset;
+  }
+  return XT_OK;
+}
+
+static int seam_upload_sino(xt_icd_state *s, float **dst, const void *src) {
+  XT_TRY(dev_alloc(dst, s->nsino));
+  XT_HIP(hipMemcpy(*dst, src, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+  return XT_OK;
+}
+
+static int seam_download_sel(xt_icd_state *s, Sinogram *S) {
+  uint8_t *sel = nullptr;
+  XT_TRY(dev_alloc(&sel, s->nsino));
+  hipLaunchKernelGGL(k_select, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->e, s->w,
+                     s->nsino, sel);
+  hipError_t e = hipStreamSynchronize(s->st);
+  if (e == hipSuccess) e = hipMemcpy(&S->ProjSelect[0][0][0], sel, s->nsino, hipMemcpyDeviceToHost);
+  (void)hipFree(sel);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "%s", hipGetErrorString(e));
+  return XT_OK;
+}
+
+extern "C" int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r) {
+  if (!S || !O || !H_r) return set_err(XT_E_ARG, "null argument");
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, T));
+  xt_icd_state *s = c.s;
+  XT_TRY(check_angles(S));
+  XT_TRY(dev_alloc(&s->viewang, S->N_p));
+  XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
+  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_TRY(launch_detector_response(s, O->BeamWidth));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_HIP(hipMemcpy(&H_r[0][0], s->Hr, (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
+                   hipMemcpyDeviceToHost));
+  return XT_OK;
+}
+
+__global__ void k_amatrix_batch(XtGeom g, const float *__restrict__ Hr, const float *__restrict__ c,
+                                const float *__restrict__ s, int n, const int *__restrict__ rows,
+                                const int *__restrict__ cols, const int *__restrict__ views,
+                                int max_entries, uint8_t *__restrict__ count,
+                                int *__restrict__ index, float *__restrict__ values) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = views[i];
+  int first;
+  float vals[XT_MAXA];
+  int cnt = xt_amatrix_column(g, Hr + (size_t)v * XT_HR_STRIDE, c[v], s[v], rows[i], cols[i], &first,
+                              vals);
+  int k = 0;
+  for (int b = 0; b < XT_MAXA; b++)
+    if (vals[b] > 0.f && k < max_entries) {
+      index[(size_t)i * max_entries + k] = first + b;
+      values[(size_t)i * max_entries + k] = vals[b];
+      k++;
+    }
+  count[i] = (uint8_t)cnt;
+}
+
+extern "C" int xt_amatrix_columns(Sinogram *S, ScannedObject *O, Real_t **H_r, int32_t n,
+                                  const int32_t *rows, const int32_t *cols, const int32_t *views,
+                                  int32_t max_entries, uint8_t *count, int32_t *index,
+                                  Real_t *values) {
+  if (!S || !O || !H_r || n < 0 || max_entries < 1) return set_err(XT_E_ARG, "bad argument");
+  if (n == 0) return XT_OK;
+  for (int32_t i = 0; i < n; i++)
+    if (views[i] < 0 || views[i] >= S->N_p || rows[i] < 0 || rows[i] >= O->N_y || cols[i] < 0 ||
+        cols[i] >= O->N_x)
+      return set_err(XT_E_ARG, "triple %d out of range", i);
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, nullptr));
+  xt_icd_state *s = c.s;
+  int *dr = nullptr, *dc = nullptr, *dv = nullptr, *di = nullptr;
+  uint8_t *dn = nullptr;
+  float *dval = nullptr;
+  XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
+  XT_TRY(dev_alloc(&s->cosv, S->N_p));
+  XT_TRY(dev_alloc(&s->sinv, S->N_p));
+  XT_HIP(hipMemcpy(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
+                   hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->cosv, S->cosine, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->sinv, S->sine, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  int rc = XT_OK;
+  if ((rc = dev_alloc(&dr, n)) || (rc = dev_alloc(&dc, n)) || (rc = dev_alloc(&dv, n)) ||
+      (rc = dev_alloc(&dn, n)) || (rc = dev_alloc(&di, (size_t)n * max_entries)) ||
+      (rc = dev_alloc(&dval, (size_t)n * max_entries))) {
+  } else {
+    hipError_t e = hipMemcpy(dr, rows, n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dc, cols, n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dv, views, n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(di, 0, (size_t)n * max_entries * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(dval, 0, (size_t)n * max_entries * sizeof(float));
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_amatrix_batch, dim3((n + 255) / 256), dim3(256), 0, s->st, s->g, s->Hr,
+                         s->cosv, s->sinv, n, dr, dc, dv, max_entries, dn, di, dval);
+      e = hipStreamSynchronize(s->st);
+    }
+    if (e == hipSuccess) e = hipMemcpy(count, dn, n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(index, di, (size_t)n * max_entries * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(values, dval, (size_t)n * max_entries * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(XT_E_HIP, "amatrix seam: %s", hipGetErrorString(e));
+  }
+  void *b[] = {dr, dc, dv, dn, di, dval};
+  for (void *p : b)
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+static int seam_geometry_upload(xt_icd_state *s, Sinogram *S, ScannedObject *O, Real_t **H_r) {
+  XT_TRY(upload_views(s, S, O));
+  XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
+  XT_HIP(hipMemcpy(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
+                   hipMemcpyHostToDevice));
+  XT_TRY(dev_alloc(&s->x, s->nobj));
+  XT_TRY(upload_object(s, O, s->x));
+  return XT_OK;
+}
+
+extern "C" int xt_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, double *AX64) {
+  if (!S || !O || !H_r || !AX64) return set_err(XT_E_ARG, "null argument");
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, nullptr));
+  xt_icd_state *s = c.s;
+  XT_TRY(seam_geometry_upload(s, S, O, H_r));
+  double *out = nullptr;
+  XT_TRY(dev_alloc(&out, s->nsino));
+  int rc = launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 0, out,
+                          nullptr, nullptr, nullptr);
+  if (rc == XT_OK) {
+    hipError_t e = hipStreamSynchronize(s->st);
+    if (e == hipSuccess) e = hipMemcpy(AX64, out, s->nsino * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(XT_E_HIP, "forward projection: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(out);
+  return rc;
+}
+
+extern "C" int xt_init_error_sino(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r,
+                                  Real_t ***E) {
+  if (!S || !O || !T || !H_r || !E) return set_err(XT_E_ARG, "null argument");
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, T));
+  xt_icd_state *s = c.s;
+  XT_TRY(seam_geometry_upload(s, S, O, H_r));
+  XT_TRY(seam_upload_sino(s, &s->y, &S->Projection[0][0][0]));
+  XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  XT_TRY(dev_alloc(&s->e, s->nsino));
+  XT_TRY(dev_alloc(&s->d, (size_t)S->N_r * S->N_t));
+  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t * sizeof(float),
+                   hipMemcpyHostToDevice));
+  XT_TRY(icd_resync(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  return seam_download_sel(s, S);
+}
+
+extern "C" int xt_compute_cost(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                               double *cost) {
+  if (!S || !O || !T || !E || !cost) return set_err(XT_E_ARG, "null argument");
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, T));
+  xt_icd_state *s = c.s;
+  XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
+  XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  XT_TRY(dev_alloc(&s->x, s->nobj));
+  XT_TRY(upload_object(s, O, s->x));
+  XT_TRY(dev_alloc(&s->red_wg, 8192));
+  XT_TRY(dev_alloc(&s->scal, 8));
+  return xt_icd_cost(s, cost);
+}
+
+extern "C" int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
+  if (!S || !T || !E) return set_err(XT_E_ARG, "null argument");
+  ScannedObject O;
+  memset(&O, 0, sizeof O);
+  O.N_x = O.N_y = 1;
+  O.N_z = S->N_t;
+  O.N_time = 1;
+  int32_t pn = 0;
+  O.ProjNum = &pn;
+  O.delta_xy = S->delta_r;
+  S->z_overlap_num = S->z_overlap_num ? S->z_overlap_num : 1;
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, &O, T));
+  xt_icd_state *s = c.s;
+  XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
+  XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  const int NRT = S->N_r * S->N_t;
+  XT_TRY(dev_alloc(&s->d, NRT));
+  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], NRT * sizeof(float), hipMemcpyHostToDevice));
+  int nch = (262144 + NRT - 1) / NRT;
+  if (nch > S->N_p) nch = S->N_p;
+  s->off_chunk = (S->N_p + nch - 1) / nch;
+  s->off_nch = (S->N_p + s->off_chunk - 1) / s->off_chunk;
+  s->off_nblk = (NRT + 255) / 256;
+  XT_TRY(dev_alloc(&s->off_part, (size_t)s->off_nch * 2 * NRT));
+  XT_TRY(dev_alloc(&s->off_numden, (size_t)2 * NRT));
+  XT_TRY(dev_alloc(&s->off_wg, (size_t)2 * s->off_nblk));
+  XT_TRY(dev_alloc(&s->scal, 8));
+  XT_TRY(offset_update(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  XT_HIP(hipMemcpy(&S->ProjOffset[0][0], s->d, NRT * sizeof(float), hipMemcpyDeviceToHost));
+  return seam_download_sel(s, S);
+}
+
+extern "C" int xt_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
+  if (!S || !T || !E) return set_err(XT_E_ARG, "null argument");
+  ScannedObject O;
+  memset(&O, 0, sizeof O);
+  O.N_x = O.N_y = 1;
+  O.N_z = S->N_t;
+  O.N_time = 1;
+  int32_t pn = 0;
+  O.ProjNum = &pn;
+  O.delta_xy = S->delta_r;
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, &O, T));
+  xt_icd_state *s = c.s;
+  XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
+  XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  XT_TRY(dev_alloc(&s->red_wg, 8192));
+  XT_TRY(dev_alloc(&s->scal, 8));
+  XT_TRY(variance_update(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  T->var_est = s->var_est;
+  return seam_download_sel(s, S);
+}
+
+extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap) {
+  if (!O || !T || !MagUpdateMap) return set_err(XT_E_ARG, "null argument");
+  int dev = 0;
+  XT_TRY(init_device(&dev));
+  const int NY = O->N_y, NX = O->N_x, nb = T->num_z_blocks, M = O->N_time * nb;
+  const size_t NYX = (size_t)NY * NX;
+  std::vector<int> ns(M);
+  for (int i = 0; i < O->N_time; i++)
+    for (int b = 0; b < nb; b++)
+      ns[i * nb + b] = (int)floor((double)(T->UpdateSelectNum[i][b] / O->NHICD_Iterations));
+  float *mag = nullptr, *filt = nullptr;
+  int *nsel = nullptr, *nout = nullptr;
+  uint8_t *flag = nullptr;
+  int rc = XT_OK;
+  std::vector<uint8_t> hflag((size_t)M * NYX);
+  if ((rc = dev_alloc(&mag, M * NYX)) || (rc = dev_alloc(&filt, M * NYX)) ||
+      (rc = dev_alloc(&nsel, M)) || (rc = dev_alloc(&nout, M)) || (rc = dev_alloc(&flag, M * NYX))) {
+  } else {
+    hipError_t e = hipMemcpy(mag, &MagUpdateMap[0][0][0][0], M * NYX * sizeof(float),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(nsel, ns.data(), M * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((NYX + 255) / 256), M), dim3(256), 0, 0, NY,
+                         NX, mag, filt);
+      hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, 0, (int)NYX, filt, nsel, flag, nout);
+      e = hipDeviceSynchronize();
+    }
+    if (e == hipSuccess) e = hipMemcpy(hflag.data(), flag, M * NYX, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(XT_E_HIP, "vsc seam: %s", hipGetErrorString(e));
+  }
+  void *b[] = {mag, filt, nsel, nout, flag};
+  for (void *p : b)
+    if (p) (void)hipFree(p);
+  if (rc) return rc;
+  for (int i = 0; i < O->N_time; i++)
+    for (int bb = 0; bb < nb; bb++) {
+      const uint8_t *F = hflag.data() + (size_t)(i * nb + bb) * NYX;
+      int idx = 0;
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NX; k++)
+          if (F[(size_t)j * NX + k]) {
+            T->x_NHICD_select[i][bb][idx] = k;
+            T->y_NHICD_select[i][bb][idx] = j;
+            idx++;
+          }
+      T->NHICDSelectNum[i][bb] = idx;
+    }
+  return XT_OK;
+}
+
+extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                               int32_t frame, int32_t n_lines, const int32_t *x_list,
+                               const int32_t *y_list, int32_t Iter, Real_t *MagUpdate,
+                               double *total_vox_mag) {
+  if (!S || !O || !T || !E || n_lines < 0 || (n_lines && (!x_list || !y_list)))
+    return set_err(XT_E_ARG, "bad argument");
+  if (frame < 0 || frame >= O->N_time) return set_err(XT_E_ARG, "frame %d", frame);
+  if (n_lines > 256) return set_err(XT_E_ARG, "at most 256 lines per call");
+  SeamCtx c;
+  XT_TRY(seam_state(c, S, O, T));
+  xt_icd_state *s = c.s;
+  const XtGeom &g = s->g;
+  if (icd_lds_bytes(g, s->ZL) > 64 * 1024) return set_err(XT_E_ARG, "too many views");
+  std::vector<int> cols(n_lines);
+  for (int i = 0; i < n_lines; i++) {
+    if (x_list[i] < 0 || x_list[i] >= g.N_x || y_list[i] < 0 || y_list[i] >= g.N_y)
+      return set_err(XT_E_ARG, "line %d out of range", i);
+    cols[i] = y_list[i] * g.N_x + x_list[i];
+  }
+  // one tile of colour 0 holding the lines in list order, updated with every
+  // line "selected" (NHICD mode walks the tile list in order)
+  int start[2] = {0, n_lines}, tbc[4] = {0, 0, 0, 0}, ntc[4] = {1, 0, 0, 0};
+  s->max_tiles = 1;
+  XT_TRY(dev_alloc(&s->tiles_by_color, 4));
+  XT_TRY(dev_alloc(&s->n_tiles, 4));
+  XT_TRY(dev_alloc(&s->tile_start, 2));
+  XT_TRY(dev_alloc(&s->tile_cols, n_lines ? n_lines : 1));
+  XT_HIP(hipMemcpy(s->tiles_by_color, tbc, sizeof tbc, hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->n_tiles, ntc, sizeof ntc, hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->tile_start, start, sizeof start, hipMemcpyHostToDevice));
+  if (n_lines)
+    XT_HIP(hipMemcpy(s->tile_cols, cols.data(), n_lines * sizeof(int), hipMemcpyHostToDevice));
+  XT_TRY(upload_views(s, S, O));
+  XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
+  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_TRY(launch_detector_response(s, O->BeamWidth));
+  XT_TRY(dev_alloc(&s->x, s->nobj));
+  XT_TRY(upload_object(s, O, s->x));
+  XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
+  XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  XT_TRY(dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX));
+  XT_HIP(hipMemset(s->nhicd_flag, 1, (size_t)g.N_time * s->NYX));
+  XT_TRY(dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX));
+  XT_TRY(dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX));
+  XT_HIP(hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)));
+  XT_TRY(dev_alloc(&s->stats, 8));
+  XT_HIP(hipMemset(s->stats, 0, 8 * sizeof(double)));
+  XtIcdLaunch L;
+  L.mode = 1;
+  L.phase = (4 - (frame & 3)) & 3;
+  L.sub_iter = 0;
+  L.K = 1;
+  L.iter = Iter;
+  L.seed = 0;
+  L.count_stats = 1;
+  L.zorder = 1;
+  XT_TRY(launch_icd(s, L));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  XT_TRY(download_object(s, O));
+  if (MagUpdate)
+    XT_HIP(hipMemcpy(MagUpdate, s->magsum + (size_t)frame * s->NYX, s->NYX * sizeof(float),
+                     hipMemcpyDeviceToHost));
+  double st[3];
+  XT_HIP(hipMemcpy(st, s->stats, sizeof st, hipMemcpyDeviceToHost));
+  if (total_vox_mag) *total_vox_mag = st[0];
+  return XT_OK;
+}

@@ -1,0 +1,331 @@
+"""ctypes binding of libxt_hip.so (include/xt_icd.h).
+
+Mirrors the reference's engine interface: ``ICD_BackProject(S, O, T)`` over the
+Source_Code_4D_Fast/XT_Structures.h structs, the engine set-up around it, and
+one seam per hot-path function.  There is no CPU fallback: loading fails with
+an error when the library has not been built, and every call raises
+``XtError`` with the library's message when it returns non-zero.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import xt_abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxt_hip.so")
+
+OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_TILE, OPT_DEVICE, OPT_VERBOSE = range(1, 8)
+
+# every symbol include/xt_icd.h declares
+EXPORTS = [
+    "xt_last_error", "xt_version", "ICD_BackProject", "xt_set_option", "xt_get_option",
+    "xt_icd_create", "xt_icd_iterate", "xt_icd_set_timing", "xt_icd_get_stats", "xt_icd_cost",
+    "xt_icd_resync", "xt_icd_download", "xt_icd_synchronize", "xt_icd_destroy",
+    "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
+    "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
+    "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
+    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines",
+]
+
+
+class XtError(RuntimeError):
+    pass
+
+
+class IcdStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("voxel_updates", C.c_double),
+                ("footprint_entries", C.c_double), ("icd_kernel_ms", C.c_double),
+                ("icd_launches", C.c_int64), ("last_cost", C.c_double),
+                ("stop_percentage", C.c_double), ("stopped", C.c_int32)]
+
+
+Sinogram, ScannedObject, AMatrixCol, TomoInputs = xt_abi.abi_types(False)
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise XtError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.xt_last_error.restype = C.c_char_p
+        L.xt_version.restype = C.c_char_p
+        L.ICD_BackProject.argtypes = [P, P, P]
+        L.xt_set_option.argtypes = [C.c_int, C.c_double]
+        L.xt_get_option.argtypes = [C.c_int]
+        L.xt_get_option.restype = C.c_double
+        L.xt_icd_create.argtypes = [P, P, P, C.POINTER(P)]
+        L.xt_icd_iterate.argtypes = [P, C.c_int32, C.POINTER(C.c_int32)]
+        L.xt_icd_set_timing.argtypes = [P, C.c_int32]
+        L.xt_icd_get_stats.argtypes = [P, C.POINTER(IcdStats)]
+        L.xt_icd_cost.argtypes = [P, C.POINTER(C.c_double)]
+        L.xt_icd_resync.argtypes = [P]
+        L.xt_icd_download.argtypes = [P, P, P, P]
+        L.xt_icd_synchronize.argtypes = [P]
+        L.xt_icd_destroy.argtypes = [P]
+        L.xt_icd_destroy.restype = None
+        L.xt_engine_setup.argtypes = [C.c_char_p, C.c_int, P, P, P, P]
+        L.xt_engine_read_sinogram.argtypes = [C.c_char_p, P, P]
+        L.xt_engine_write_outputs.argtypes = [C.c_char_p, P, P, P]
+        L.xt_engine_free.argtypes = [P, P, P]
+        L.xt_engine_free.restype = None
+        L.xt_engine_run.argtypes = [C.c_char_p, C.c_int, P]
+        L.xt_multialloc.argtypes = [C.c_size_t, C.c_int, P]
+        L.xt_multialloc.restype = P
+        L.xt_multifree.argtypes = [P, C.c_int]
+        L.xt_multifree.restype = None
+        L.xt_detector_response.argtypes = [P, P, P, P]
+        L.xt_amatrix_columns.argtypes = [P, P, P, C.c_int32, P, P, P, C.c_int32, P, P, P]
+        L.xt_forward_project.argtypes = [P, P, P, P]
+        L.xt_init_error_sino.argtypes = [P, P, P, P, P]
+        L.xt_compute_cost.argtypes = [P, P, P, P, C.POINTER(C.c_double)]
+        L.xt_offset_update.argtypes = [P, P, P]
+        L.xt_variance_update.argtypes = [P, P, P]
+        L.xt_vsc_select.argtypes = [P, P, P]
+        L.xt_update_lines.argtypes = [P, P, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P,
+                                      C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise XtError(f"libxt_hip error {rc}: {lib().xt_last_error().decode()}")
+
+
+def version() -> str:
+    return lib().xt_version().decode()
+
+
+def set_option(opt: int, value: float) -> None:
+    _check(lib().xt_set_option(opt, float(value)))
+
+
+def get_option(opt: int) -> float:
+    return lib().xt_get_option(opt)
+
+
+def _argv(args):
+    return len(args), (C.c_char_p * max(1, len(args)))(*[a.encode() for a in args])
+
+
+# -- engine -----------------------------------------------------------------
+class Engine:
+    """Structs set up by the library from a run folder (the reference main's
+    argsParser + initStructures + genSinogram_fromBin)."""
+
+    def __init__(self, path: str, args, read_sinogram: bool = True):
+        self.path = path
+        self.S, self.O, self.T = Sinogram(), ScannedObject(), TomoInputs()
+        n, arr = _argv(args)
+        rc = lib().xt_engine_setup(path.encode(), n, arr, C.byref(self.S), C.byref(self.O),
+                                   C.byref(self.T))
+        if rc != 0:
+            lib().xt_engine_free(C.byref(self.S), C.byref(self.O), C.byref(self.T))
+            _check(rc)
+        if read_sinogram:
+            _check(lib().xt_engine_read_sinogram(path.encode(), C.byref(self.S), C.byref(self.T)))
+
+    def refs(self):
+        return C.byref(self.S), C.byref(self.O), C.byref(self.T)
+
+    def close(self):
+        if self.S is not None:
+            lib().xt_engine_free(*self.refs())
+            self.S = self.O = self.T = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # array views
+    def _view(self, ptr, levels, ctype, n):
+        addr = ptr
+        for _ in range(levels):
+            addr = C.cast(addr, C.POINTER(C.c_void_p))[0]
+        return np.ctypeslib.as_array((ctype * n).from_address(addr))
+
+    def object(self, frame: int) -> np.ndarray:
+        """Object[frame] as [N_z+2][N_y][N_x] (halo slices included)."""
+        O = self.O
+        top = C.cast(O.Object, C.POINTER(C.c_void_p))[frame]
+        return self._view(top, 2, C.c_float, (O.N_z + 2) * O.N_y * O.N_x).reshape(
+            O.N_z + 2, O.N_y, O.N_x)
+
+    def volume(self) -> np.ndarray:
+        return np.stack([self.object(f)[1:self.O.N_z + 1].copy() for f in range(self.O.N_time)])
+
+    def set_volume(self, vol) -> None:
+        for f in range(self.O.N_time):
+            v = self.object(f)
+            v[:] = 0
+            v[1:self.O.N_z + 1] = vol[f]
+
+    def sino(self, which: str) -> np.ndarray:
+        S, T = self.S, self.T
+        n = S.N_p * S.N_r * S.N_t
+        ptr = {"Projection": S.Projection, "Weight": T.Weight}[which]
+        return self._view(ptr, 2, C.c_float, n).reshape(S.N_p, S.N_r, S.N_t)
+
+    def proj_select(self) -> np.ndarray:
+        S = self.S
+        return self._view(S.ProjSelect, 2, C.c_uint8, S.N_p * S.N_r * S.N_t).reshape(
+            S.N_p, S.N_r, S.N_t)
+
+    def offset(self) -> np.ndarray:
+        S = self.S
+        return self._view(S.ProjOffset, 1, C.c_float, S.N_r * S.N_t).reshape(S.N_r, S.N_t)
+
+
+def engine_run(path: str, args) -> None:
+    """The whole reference engine on a run folder (object_n0_time_<i>.bin out)."""
+    n, arr = _argv(args)
+    _check(lib().xt_engine_run(path.encode(), n, arr))
+
+
+def ICD_BackProject(S, O, T) -> None:
+    _check(lib().ICD_BackProject(C.byref(S), C.byref(O), C.byref(T)))
+
+
+# -- stepwise ---------------------------------------------------------------
+class Icd:
+    """xt_icd_create / iterate / download over an Engine's structs."""
+
+    def __init__(self, eng: Engine):
+        self.eng = eng
+        self.h = C.c_void_p()
+        _check(lib().xt_icd_create(*eng.refs(), C.byref(self.h)))
+
+    def iterate(self, n: int = 1) -> bool:
+        stop = C.c_int32(0)
+        _check(lib().xt_icd_iterate(self.h, n, C.byref(stop)))
+        return bool(stop.value)
+
+    def timing(self, on: bool) -> None:
+        _check(lib().xt_icd_set_timing(self.h, 1 if on else 0))
+
+    def stats(self) -> IcdStats:
+        st = IcdStats()
+        _check(lib().xt_icd_get_stats(self.h, C.byref(st)))
+        return st
+
+    def cost(self) -> float:
+        c = C.c_double()
+        _check(lib().xt_icd_cost(self.h, C.byref(c)))
+        return c.value
+
+    def resync(self) -> None:
+        _check(lib().xt_icd_resync(self.h))
+
+    def synchronize(self) -> None:
+        _check(lib().xt_icd_synchronize(self.h))
+
+    def download(self) -> None:
+        _check(lib().xt_icd_download(self.h, *self.eng.refs()))
+
+    def close(self) -> None:
+        if self.h:
+            lib().xt_icd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# -- seams ------------------------------------------------------------------
+def detector_response(eng: Engine) -> np.ndarray:
+    H = np.zeros((eng.S.N_p, 65), np.float32)
+    tree = xt_abi.PtrTree(H)
+    _check(lib().xt_detector_response(*eng.refs(), tree.ptr))
+    return H
+
+
+def amatrix_columns(eng: Engine, H, rows, cols, views, max_entries=8):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    views = np.ascontiguousarray(views, np.int32)
+    n = rows.size
+    cnt = np.zeros(n, np.uint8)
+    idx = np.zeros((n, max_entries), np.int32)
+    val = np.zeros((n, max_entries), np.float32)
+    ht = xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    _check(lib().xt_amatrix_columns(C.byref(eng.S), C.byref(eng.O), ht.ptr, n,
+                                    rows.ctypes.data, cols.ctypes.data, views.ctypes.data,
+                                    max_entries, cnt.ctypes.data, idx.ctypes.data,
+                                    val.ctypes.data))
+    return cnt, idx, val
+
+
+def forward_project(eng: Engine, H) -> np.ndarray:
+    S = eng.S
+    out = np.zeros((S.N_p, S.N_r, S.N_t), np.float64)
+    ht = xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    _check(lib().xt_forward_project(C.byref(eng.S), C.byref(eng.O), ht.ptr, out.ctypes.data))
+    return out
+
+
+def init_error_sino(eng: Engine, H) -> np.ndarray:
+    S = eng.S
+    E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
+    et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    _check(lib().xt_init_error_sino(*eng.refs(), ht.ptr, et.ptr))
+    return E
+
+
+def compute_cost(eng: Engine, E) -> float:
+    et = xt_abi.PtrTree(np.ascontiguousarray(E, np.float32))
+    c = C.c_double()
+    _check(lib().xt_compute_cost(*eng.refs(), et.ptr, C.byref(c)))
+    return c.value
+
+
+def offset_update(eng: Engine, E: np.ndarray) -> None:
+    et = xt_abi.PtrTree(E)
+    _check(lib().xt_offset_update(C.byref(eng.S), C.byref(eng.T), et.ptr))
+
+
+def variance_update(eng: Engine, E: np.ndarray) -> None:
+    et = xt_abi.PtrTree(E)
+    _check(lib().xt_variance_update(C.byref(eng.S), C.byref(eng.T), et.ptr))
+
+
+def vsc_select(eng: Engine, mag: np.ndarray, update_select_num: np.ndarray):
+    """mag [N_time][nb][N_y][N_x]; returns per (frame, block) lists of (y, x)."""
+    O, T = eng.O, eng.T
+    nb = T.num_z_blocks
+    usn = eng._view(T.UpdateSelectNum, 1, C.c_int32, O.N_time * nb).reshape(O.N_time, nb)
+    usn[:] = update_select_num
+    mt = xt_abi.PtrTree(np.ascontiguousarray(mag, np.float32))
+    _check(lib().xt_vsc_select(C.byref(eng.O), C.byref(eng.T), mt.ptr))
+    nxy = O.N_y * O.N_x
+    xs = eng._view(T.x_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
+    ys = eng._view(T.y_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
+    num = eng._view(T.NHICDSelectNum, 1, C.c_int32, O.N_time * nb).reshape(O.N_time, nb)
+    return [[(ys[i, b, :num[i, b]].copy(), xs[i, b, :num[i, b]].copy()) for b in range(nb)]
+            for i in range(O.N_time)]
+
+
+def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int):
+    """Sequential update of the listed lines; returns (MagUpdate [N_y][N_x], total_vox_mag)."""
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    O = eng.O
+    mag = np.zeros((O.N_y, O.N_x), np.float32)
+    tot = C.c_double()
+    et = xt_abi.PtrTree(E)
+    _check(lib().xt_update_lines(*eng.refs(), et.ptr, frame, xs.size, xs.ctypes.data,
+                                 ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot)))
+    return mag, tot.value
