@@ -111,6 +111,9 @@ class Oracle:
         L.oracle_interlaced_views.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                               C.c_double, P, P, P, P, P]
         L.oracle_interlaced_views.restype = C.c_int32
+        L.oracle_sample_updates.argtypes = [P, P, P, P, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.oracle_sample_updates.restype = C.c_int64
 
     # -- engine -----------------------------------------------------------
     def setup(self, path, args):

@@ -899,6 +899,70 @@ static int32_t icd_iteration(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_
   return (avg_update_percentage < T->StopThreshold) ? 1 : 0;
 }
 
+/* Timed sample of the reference's hot loop for a CPU baseline: one homogeneous
+ * half-phase of updateVoxelsTimeSlices (F/XT_ICD_update.c:799-813) -- OpenMP
+ * over (frame, z-block of the frame's parity) with num_z_blocks as the
+ * reference derives it from the thread count -- where every task updates the
+ * first lines_per_block lines of a random order, all slices of its block.
+ * E must hold the error sinogram.  Returns performed voxel updates; the wall
+ * time goes to *seconds and the number of concurrent tasks to *tasks. */
+int64_t oracle_sample_updates(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                              int32_t lines_per_block, int32_t n_threads, double *seconds,
+                              int32_t *tasks) {
+  int32_t nb = T->num_z_blocks;
+  Real_t **H_r = (Real_t **)alloc2(sizeof(Real_t), S->N_p, XT_DETECTOR_RESPONSE_BINS + 1);
+  oracle_detector_response(H_r, S, O, T);
+  uint8_t ****Mask = (uint8_t ****)alloc4(sizeof(uint8_t), O->N_time, O->N_y, O->N_x, O->N_z);
+  memset(&Mask[0][0][0][0], 1, (size_t)O->N_time * O->N_y * O->N_x * O->N_z);
+  Real_t ****Mag = (Real_t ****)alloc4(sizeof(Real_t), O->N_time, nb, O->N_y, O->N_x);
+  oracle_random_select(O, T, Mask);
+  int64_t *zc = (int64_t *)o_calloc((size_t)O->N_time * nb, sizeof(int64_t));
+  int32_t ntask = 0;
+  for (int32_t i = 0; i < O->N_time; i++)
+    for (int32_t block = 0; block < nb; block += 2) ntask++;
+  *tasks = ntask;
+#ifdef _OPENMP
+  omp_set_num_threads(n_threads);
+  double t0 = omp_get_wtime();
+#else
+  (void)n_threads;
+  double t0 = 0;
+#endif
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int32_t i = 0; i < O->N_time; i++)
+    for (int32_t block = 0; block < nb; block += 2) {
+      int32_t idx = (i % 2 == 0) ? block : block + 1;
+      if (idx >= nb) continue;
+      int32_t z_start = idx * (O->N_z / nb);
+      int32_t z_stop = (idx >= nb - 1) ? O->N_z - 1 : (idx + 1) * (O->N_z / nb) - 1;
+      int32_t n = lines_per_block < T->UpdateSelectNum[i][idx] ? lines_per_block
+                                                                 : T->UpdateSelectNum[i][idx];
+      update_lines_masked(S, O, T, E, H_r, i, z_start, z_stop, n, T->x_rand_select[i][idx],
+                          T->y_rand_select[i][idx], 1, &zc[i * nb + idx], Mag[i][idx], Mask[i], 0);
+    }
+#ifdef _OPENMP
+  *seconds = omp_get_wtime() - t0;
+#else
+  *seconds = 0;
+#endif
+  int64_t upd = 0;
+  for (int32_t i = 0; i < O->N_time; i++)
+    for (int32_t block = 0; block < nb; block += 2) {
+      int32_t idx = (i % 2 == 0) ? block : block + 1;
+      if (idx >= nb) continue;
+      int32_t z_start = idx * (O->N_z / nb);
+      int32_t z_stop = (idx >= nb - 1) ? O->N_z - 1 : (idx + 1) * (O->N_z / nb) - 1;
+      int32_t n = lines_per_block < T->UpdateSelectNum[i][idx] ? lines_per_block
+                                                                 : T->UpdateSelectNum[i][idx];
+      upd += (int64_t)n * (z_stop - z_start + 1) - zc[i * nb + idx];
+    }
+  free(zc);
+  oracle_multifree(Mag, 4);
+  oracle_multifree(Mask, 4);
+  oracle_multifree(H_r, 2);
+  return upd;
+}
+
 static int32_t write_bin(const char *dir, const char *name, const void *data, size_t n) {
   char path[1024];
   snprintf(path, sizeof path, "%s%s%s.bin", dir ? dir : "", (dir && dir[0]) ? "/" : "", name);

@@ -69,6 +69,12 @@ Real_t oracle_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t 
 void oracle_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap);
 void oracle_random_select(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
 
+/* Timed sample of one homogeneous half-phase of the reference's schedule (the
+ * benchmark's CPU baseline); see xt_oracle.c. */
+int64_t oracle_sample_updates(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                              int32_t lines_per_block, int32_t n_threads, double *seconds,
+                              int32_t *tasks);
+
 /* Full driver: same semantics as ICD_BackProject.  flags: bit0 compute the MAP
  * cost every iteration (the reference's build without NO_COST_CALCULATE),
  * bit1 positivity constraint.  cost_out (may be NULL) receives NumIter+1

@@ -66,7 +66,7 @@ def make_scan(cfg, tmp_root, tag=""):
     """Generate cfg into a fresh folder (cached per session) and return a copy path."""
     key = (cfg.name, cfg.seed, cfg.N_xy, cfg.N_z, cfg.N_p, cfg.N_time)
     if key not in _scan_cache:
-        src = os.path.join(str(tmp_root), f"scan_{cfg.name}_{cfg.seed}")
+        src = os.path.join(str(tmp_root), "scan_" + "_".join(str(k) for k in key))
         o64 = py_oracle.Oracle(True, float_amatrix=True)
         info = synth.generate(cfg, src, o64.projector(src, cfg.cli_args()))
         _scan_cache[key] = (src, info)

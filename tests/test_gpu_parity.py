@@ -243,11 +243,13 @@ def test_vsc_select_bitexact(scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
+@pytest.mark.parametrize("atable", [1, 0], ids=["atable", "onthefly"])
 @pytest.mark.parametrize("Iter", [1, 2])
-def test_update_lines_per_voxel(Iter, scan_root, oracle32):
+def test_update_lines_per_voxel(Iter, atable, scan_root, oracle32):
     cfg = TINY
-    _print_case(f"update_lines Iter={Iter}", cfg)
+    _print_case(f"update_lines Iter={Iter} atable={atable}", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    xt_hip.set_option(xt_hip.OPT_ATABLE, atable)
     try:
         vol = (0.7 * info["phantom"]).astype(np.float32)
         vol[:, :, :4, :] = 0  # zero rows: zero-skipping paths
@@ -279,5 +281,6 @@ def test_update_lines_per_voxel(Iter, scan_root, oracle32):
         assert np.abs(mag_g - mag_o).max() <= 1e-4 * max(1e-12, np.abs(mag_o).max())
         assert abs(tot_g - tot_o) <= 1e-4 * max(1e-12, abs(tot_o))
     finally:
+        xt_hip.set_option(xt_hip.OPT_ATABLE, 1)
         eng.close()
         oracle32.free(S, O, T)

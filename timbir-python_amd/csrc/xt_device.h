@@ -293,7 +293,49 @@ struct XtIcdBuffers {
   float *mag_blk;            // [N_time][nb][N_y][N_x]
   float *magsum;             // [N_time][N_y][N_x]
   double *stats;             // [0] total_vox_mag, [1] voxel updates, [2] footprint entries
+  const float4 *at_vals;     // A-column table [N_time][N_y*N_x][max_views] (or null)
+  const int *at_first;       // first bin of each table entry
 };
+
+// ---------------------------------------------------------------------------
+// A-column table: the exact columns of every (frame, line, view), built once
+// per reconstruction and read by the voxel update instead of rebuilding each
+// column per line visit.  An entry is the first non-zero bin and the next four
+// values (20 B; a column of delta_xy <= 1.5 delta_r spans at most 4 bins).
+// *overflow is set when some column needs more than four bins; the host then
+// keeps building columns on the fly.
+__global__ void k_build_atable(XtGeom g, const float *__restrict__ Hr, const float *__restrict__ cosv,
+                               const float *__restrict__ sinv, const int *__restrict__ views,
+                               const int *__restrict__ proj_num, float4 *__restrict__ at_vals,
+                               int *__restrict__ at_first, int *__restrict__ overflow) {
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= NYX * g.max_views) return;
+  const int c = (int)(i / g.max_views), p = (int)(i % g.max_views);
+  const size_t ti = (size_t)f * NYX * g.max_views + i;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  int fb = 0;
+  if (p < proj_num[f]) {
+    const int v = views[(size_t)f * g.max_views + p];
+    const int row = c / g.N_x, col = c - row * g.N_x;
+    int first;
+    float vals[XT_MAXA];
+    xt_amatrix_column(g, Hr + (size_t)v * XT_HR_STRIDE, cosv[v], sinv[v], row, col, &first, vals);
+    int b0 = 0;
+    while (b0 < XT_MAXA && !(vals[b0] > 0.f)) b0++;
+    if (b0 < XT_MAXA) {
+      fb = first + b0;
+      float t[4];
+      for (int k = 0; k < 4; k++) t[k] = (b0 + k < XT_MAXA) ? vals[b0 + k] : 0.f;
+      for (int k = b0 + 4; k < XT_MAXA; k++)
+        if (vals[k] > 0.f) atomicOr(overflow, 1);
+      out = make_float4(t[0], t[1], t[2], t[3]);
+    }
+  }
+  at_vals[ti] = out;
+  at_first[ti] = fb;
+}
 
 __device__ __forceinline__ uint32_t xt_hash(uint32_t a) {
   a ^= a >> 16;
@@ -315,7 +357,7 @@ __device__ __forceinline__ uint32_t xt_hash(uint32_t a) {
 // Tiles of one colour are never adjacent, so concurrently updated lines are
 // not prior neighbours; the colour is shifted per frame so adjacent frames
 // never update the same line at once.
-template <int ZL>
+template <int ZL, bool TAB>
 __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
   constexpr int NT = 256;
   constexpr int G = NT / ZL;
@@ -341,6 +383,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
   const int *vlist = B.views + (size_t)f * g.max_views;
   const size_t NYX = (size_t)g.N_y * g.N_x;
   const bool zok = zl < g.N_z;
+  constexpr int NA = TAB ? 4 : XT_MAXA;  // bins a column can hold
 
   // ---- lines of the tile this launch updates, in order
   if (tid == 0) {
@@ -388,12 +431,25 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     // ---- theta pass over the footprint
     float th1 = 0.f, th2 = 0.f;
     int cnt_local = 0;
+    const size_t trow = ((size_t)f * NYX + c) * g.max_views;
     for (int p = grp; p < nv; p += G) {
       const int v = vlist[p];
-      int first;
+      int first, cnt;
       float vals[XT_MAXA];
-      const int cnt = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v],
-                                        yy, xx, &first, vals);
+      if constexpr (TAB) {
+        const float4 t4 = B.at_vals[trow + p];
+        first = B.at_first[trow + p];
+        vals[0] = t4.x;
+        vals[1] = t4.y;
+        vals[2] = t4.z;
+        vals[3] = t4.w;
+#pragma unroll
+        for (int b = 4; b < XT_MAXA; b++) vals[b] = 0.f;
+        cnt = (t4.x > 0.f) + (t4.y > 0.f) + (t4.z > 0.f) + (t4.w > 0.f);
+      } else {
+        cnt = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx,
+                                &first, vals);
+      }
       if (zl == 0) {
         a_first[p] = first;
 #pragma unroll
@@ -404,7 +460,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
         const size_t base = ((size_t)v * g.N_r + first) * g.N_t + zl;
         float ev[XT_MAXA], wv[XT_MAXA];
 #pragma unroll
-        for (int b = 0; b < XT_MAXA; b++) {
+        for (int b = 0; b < NA; b++) {
           ev[b] = 0.f;
           wv[b] = 0.f;
           if (vals[b] > 0.f) {
@@ -413,7 +469,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
           }
         }
 #pragma unroll
-        for (int b = 0; b < XT_MAXA; b++) {
+        for (int b = 0; b < NA; b++) {
           const float pe = vals[b] * g.delta_t;
           // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
           if (ev[b] * ev[b] * wv[b] < g.T * g.T) {
@@ -542,7 +598,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
           const int v = vlist[p];
           const size_t base = ((size_t)v * g.N_r + a_first[p]) * g.N_t + zl;
 #pragma unroll
-          for (int b = 0; b < XT_MAXA; b++) {
+          for (int b = 0; b < NA; b++) {
             const float a = a_vals[(size_t)p * XT_MAXA + b];
             if (a > 0.f) {
               const float pe = a * g.delta_t;

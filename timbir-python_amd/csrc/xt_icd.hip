@@ -51,19 +51,19 @@ static int set_err(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0};
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 8) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 9) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 8) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 9) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -218,6 +218,8 @@ struct xt_icd_state {
   int *nsel = nullptr, *nsel_out = nullptr;
   double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
   double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
+  float4 *at_vals = nullptr;  // A-column table (null: columns built on the fly)
+  int *at_first = nullptr;
   int off_nch = 1, off_chunk = 1, off_nblk = 1;
   // statistics
   double total_updates = 0, total_foot = 0, icd_ms = 0, last_cost = NAN, stop_pct = NAN;
@@ -233,7 +235,8 @@ static void free_state(xt_icd_state *s) {
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
                   s->tiles_by_color, s->n_tiles, s->tile_start, s->tile_cols, s->nhicd_flag,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
-                  s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg};
+                  s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
+                  s->at_vals,  s->at_first};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -274,8 +277,12 @@ static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *
 template <int ZL>
 static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
                        int gx, size_t lds) {
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL>), dim3(gx, g.N_time), dim3(256), lds, st, g, B,
-                     L);
+  if (B.at_vals)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL, true>), dim3(gx, g.N_time), dim3(256), lds,
+                       st, g, B, L);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL, false>), dim3(gx, g.N_time), dim3(256), lds,
+                       st, g, B, L);
 }
 
 static XtIcdBuffers buffers_of(xt_icd_state *s) {
@@ -297,6 +304,8 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   B.mag_blk = s->mag_blk;
   B.magsum = s->magsum;
   B.stats = s->stats;
+  B.at_vals = s->at_vals;
+  B.at_first = s->at_first;
   return B;
 }
 
@@ -517,6 +526,38 @@ static int icd_resync(xt_icd_state *s) {
                         nullptr, s->y, s->d, s->e);
 }
 
+// The A-column table is used when it fits in half of the free device memory
+// (XT_OPT_ATABLE = 1, the default) and every column spans at most four bins.
+static int build_atable(xt_icd_state *s) {
+  if (g_opt[XT_OPT_ATABLE] == 0) return XT_OK;
+  const XtGeom &g = s->g;
+  const size_t n = (size_t)g.N_time * s->NYX * g.max_views;
+  size_t free_b = 0, total_b = 0;
+  XT_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (n * (sizeof(float4) + sizeof(int)) > free_b / 2) return XT_OK;
+  int *ovf = nullptr;
+  XT_TRY(dev_alloc(&s->at_vals, n));
+  XT_TRY(dev_alloc(&s->at_first, n));
+  XT_TRY(dev_alloc(&ovf, 1));
+  XT_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s->st));
+  dim3 grid((unsigned)((s->NYX * g.max_views + 255) / 256), g.N_time);
+  hipLaunchKernelGGL(k_build_atable, grid, dim3(256), 0, s->st, g, s->Hr, s->cosv, s->sinv, s->views,
+                     s->proj_num, s->at_vals, s->at_first, ovf);
+  int h_ovf = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&h_ovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  (void)hipFree(ovf);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "A-column table: %s", hipGetErrorString(e));
+  if (h_ovf) {
+    (void)hipFree(s->at_vals);
+    (void)hipFree(s->at_first);
+    s->at_vals = nullptr;
+    s->at_first = nullptr;
+  }
+  return XT_OK;
+}
+
 static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **out) {
   *out = nullptr;
   xt_icd_state *s = new xt_icd_state();
@@ -601,6 +642,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   }
   if ((rc = upload_object(s, O, s->x))) return fail(rc);
   if ((rc = launch_detector_response(s, O->BeamWidth))) return fail(rc);
+  if ((rc = build_atable(s))) return fail(rc);
   if ((rc = icd_resync(s))) return fail(rc);
   if (hipStreamSynchronize(s->st) != hipSuccess)
     return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
@@ -798,8 +840,8 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   }
   xt_icd_state *s = nullptr;
   XT_TRY(create_impl(S, O, T, &s));
-  dbg(T, "ICD_BackProject: %s, tiles %dx%d (%d), z lanes %d\n", xt_version(), s->TS, s->TS,
-      s->n_tiles_tot, s->ZL);
+  dbg(T, "ICD_BackProject: %s, tiles %dx%d (%d), z lanes %d, A table %s\n", xt_version(), s->TS,
+      s->TS, s->n_tiles_tot, s->ZL, s->at_vals ? "on" : "off");
   // the error sinogram now lives on the GPU; Projection is released as the
   // reference does (F/XT_ICD_update.c:999)
   xt_multifree(S->Projection, 3);
@@ -1173,6 +1215,7 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
   XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
   XT_TRY(launch_detector_response(s, O->BeamWidth));
+  XT_TRY(build_atable(s));
   XT_TRY(dev_alloc(&s->x, s->nobj));
   XT_TRY(upload_object(s, O, s->x));
   XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));

@@ -18,7 +18,8 @@ from . import xt_abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
-OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_TILE, OPT_DEVICE, OPT_VERBOSE = range(1, 8)
+(OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_TILE, OPT_DEVICE, OPT_VERBOSE,
+ OPT_OUT_DIR, OPT_ATABLE) = range(1, 10)
 
 # every symbol include/xt_icd.h declares
 EXPORTS = [
@@ -136,6 +137,10 @@ class Engine:
 
     def refs(self):
         return C.byref(self.S), C.byref(self.O), C.byref(self.T)
+
+    def read_sinogram(self) -> None:
+        _check(lib().xt_engine_read_sinogram(self.path.encode(), C.byref(self.S),
+                                             C.byref(self.T)))
 
     def close(self):
         if self.S is not None:
