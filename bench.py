@@ -157,7 +157,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters for the roofline traffic figure)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py: this engine runs one slab on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU z-slab sharding is a design note only)", file=sys.stderr)

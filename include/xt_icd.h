@@ -74,9 +74,12 @@ enum {
   XT_OPT_DEVICE = 6,         /* HIP device ordinal (default 0) */
   XT_OPT_VERBOSE = 7,        /* per-iteration lines to the debug file (default 1) */
   XT_OPT_OUT_DIR = 8,        /* reserved */
-  XT_OPT_ATABLE = 9          /* 1: keep every line's A columns in an HBM table built once
+  XT_OPT_ATABLE = 9,         /* 1: keep every line's A columns in an HBM table built once
                                 (when it fits and each column spans <= 4 bins); 0: build
                                 them on every visit (default 1) */
+  XT_OPT_GRAPH = 10          /* 1: capture one iteration's launches as a hipGraph and replay
+                                it from the second iteration on (not while per-launch timing
+                                is on); 0: enqueue every launch (default 1) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);

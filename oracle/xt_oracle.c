@@ -1047,6 +1047,14 @@ int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, con
       if (cost_out) cost_out[Iter] = c;
       dbg(T, "oracle: ICD iter %d cost %f\n", Iter, c);
     }
+    if (flags & 4) { /* snapshot of every iteration's object */
+      for (i = 0; i < O->N_time; i++) {
+        char name[128];
+        snprintf(name, sizeof name, "object_n%d_time_%d_iter_%d", T->node_rank, i, Iter);
+        write_bin(dir, name, &(O->Object[i][1][0][0]), (size_t)O->N_z * O->N_y * O->N_x);
+      }
+      if (T->debug_file_ptr) fflush(T->debug_file_ptr);
+    }
     if (flag != 0 && Iter > 1) break;
   }
   {

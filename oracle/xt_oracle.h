@@ -77,7 +77,8 @@ int64_t oracle_sample_updates(Sinogram *S, ScannedObject *O, TomoInputs *T, Real
 
 /* Full driver: same semantics as ICD_BackProject.  flags: bit0 compute the MAP
  * cost every iteration (the reference's build without NO_COST_CALCULATE),
- * bit1 positivity constraint.  cost_out (may be NULL) receives NumIter+1
+ * bit1 positivity constraint, bit2 write object_n0_time_<i>_iter_<k>.bin
+ * after every iteration.  cost_out (may be NULL) receives NumIter+1
  * costs (index 0 = before ICD; unused entries NaN).  Returns iterations run. */
 int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, const char *dir,
                                int32_t flags, double *cost_out);

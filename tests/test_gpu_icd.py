@@ -98,6 +98,40 @@ def test_stepwise_and_resync(scan_root):
         assert np.all(np.isfinite(vol)) and np.abs(vol).max() > 0
 
 
+def test_graph_replay_matches_eager(scan_root):
+    """Iterations replayed from one captured hipGraph against the same iterations
+    enqueued eagerly, with eager cost evaluations and synchronisations between the
+    replays: same launches, and every replay's cost agrees with its eager
+    counterpart.  Two eager runs already differ (the order of the error-sinogram
+    atomics and of concurrent reads of it varies from run to run, and ICD
+    amplifies such differences in its first iterations), so the bar is that
+    run-to-run spread: 1e-4 after the first iteration, 5e-3 after later ones."""
+    cfg = dataclasses.replace(TINY, iterations=6)
+    path, info = make_scan(cfg, scan_root, "graph")
+    costs = {}
+    for graph in (0, 1):
+        xt_hip.set_option(xt_hip.OPT_GRAPH, graph)
+        try:
+            with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
+                c = []
+                for it in range(5):
+                    icd.iterate(1)
+                    c.append(icd.cost())
+                    icd.synchronize()
+                st = icd.stats()
+                assert st.iterations == 5
+                costs[graph] = (np.array(c), st.voxel_updates, st.icd_launches)
+        finally:
+            xt_hip.set_option(xt_hip.OPT_GRAPH, 1)
+    ce, ue, le = costs[0]
+    cg, ug, lg = costs[1]
+    print(f"  eager {ce}\n  graph {cg}")
+    assert abs(cg[0] - ce[0]) <= 1e-4 * abs(ce[0])
+    assert np.all(np.abs(cg - ce) <= 5e-3 * np.abs(ce))
+    assert np.all(np.diff(cg) < 0) and np.all(np.diff(ce) < 0)
+    assert lg == le and ug > 0 and ue > 0
+
+
 def test_icd_backproject_struct_boundary(scan_root, tmp_path):
     """The drop-in call itself: structs in, Projection freed, outputs written."""
     cfg = dataclasses.replace(TINY, iterations=3)

@@ -269,7 +269,6 @@ struct XtIcdLaunch {
   int phase;       // 0..3: frame f updates tiles of colour (phase + f) & 3
   int sub_iter;    // j of K
   int K;           // NHICD_Iterations
-  int iter;        // ICD iteration (1-based)
   uint32_t seed;
   int count_stats; // accumulate total_vox_mag (homogeneous sweeps)
   int zorder;      // 0: slices in red-black order; 1: ascending, one slice at a time
@@ -289,12 +288,15 @@ struct XtIcdBuffers {
   int max_tiles;
   const int *tile_start;     // [n_tiles_total + 1] into tile_cols
   const int *tile_cols;      // packed y*N_x + x, masked columns in tile raster order
-  const uint8_t *nhicd_flag; // [N_time][N_y][N_x]
+  const uint8_t *nhicd_flag; // [N_time][N_y][N_x]: line selected by some z-block
+  const uint8_t *nhicd_blk;  // [N_time][nb][N_y][N_x]: selected by that z-block
   float *mag_blk;            // [N_time][nb][N_y][N_x]
   float *magsum;             // [N_time][N_y][N_x]
   double *stats;             // [0] total_vox_mag, [1] voxel updates, [2] footprint entries
   const float4 *at_vals;     // A-column table [N_time][N_y*N_x][max_views] (or null)
   const int *at_first;       // first bin of each table entry
+  const int *iter;           // ICD iteration number (device memory, so a captured
+                             // graph of one iteration replays for every iteration)
 };
 
 // ---------------------------------------------------------------------------
@@ -361,9 +363,10 @@ template <int ZL, bool TAB>
 __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
   constexpr int NT = 256;
   constexpr int G = NT / ZL;
-  extern __shared__ float smem[];
-  float *a_vals = smem;                                            // [max_views][XT_MAXA]
-  int *a_first = (int *)(a_vals + (size_t)g.max_views * XT_MAXA);  // [max_views]
+  extern __shared__ float4 smem4[];
+  float4 *a_v4 = smem4;                                            // [max_views] bins 0..3
+  float2 *a_v2 = (float2 *)(a_v4 + g.max_views);                   // [max_views] bins 4..5
+  int *a_first = (int *)(a_v2 + g.max_views);                      // [max_views]
   float *red1 = (float *)(a_first + g.max_views);                  // [NT]
   float *red2 = red1 + NT;                                         // [NT]
   float *xcol = red2 + NT;                                         // [ZL]
@@ -384,6 +387,13 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
   const size_t NYX = (size_t)g.N_y * g.N_x;
   const bool zok = zl < g.N_z;
   constexpr int NA = TAB ? 4 : XT_MAXA;  // bins a column can hold
+  const int iter = *B.iter;
+  // z-block of this slice: block b holds slices [b*floor(N_z/nb), (b+1)*floor(N_z/nb)),
+  // the last one up to N_z-1 (F/XT_ICD_update.c:804-806), so with fewer slices
+  // than blocks every slice is in the last block
+  const int blk_sz = g.N_z / g.nb;
+  int zblk = blk_sz > 0 ? zl / blk_sz : g.nb - 1;
+  if (zblk >= g.nb) zblk = g.nb - 1;
 
   // ---- lines of the tile this launch updates, in order
   if (tid == 0) {
@@ -392,7 +402,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
       // a random order of the tile's lines for this iteration (the same in every
       // sub-iteration), then its j-th 1/K part as F/XT_ICD_update.c:807-809
       for (int i = 0; i < nt; i++) order[i] = B.tile_cols[t0 + i];
-      uint32_t h = xt_hash(L.seed ^ xt_hash((uint32_t)L.iter * 0x9E3779B9U ^
+      uint32_t h = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x9E3779B9U ^
                                             xt_hash((uint32_t)f * 7919U + (uint32_t)tile)));
       for (int i = nt - 1; i > 0; i--) {
         h = xt_hash(h + 0x6D2B79F5U);
@@ -423,64 +433,85 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
   for (int li = 0; li < n_lines; li++) {
     const int c = order[li];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
+    // NHICD lines: a slice is updated only if its own z-block selected the line
+    // (F/XT_ICD_update.c:844-881 walk each block's own list)
+    const bool lane_on =
+        zok && (L.mode == 0 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
     // earlier lines' e and x writes of this workgroup must be seen by its loads
     __threadfence();
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    // ---- theta pass over the footprint
+    // ---- theta pass over the footprint, two views per step so each wave keeps
+    // the loads of both in flight
     float th1 = 0.f, th2 = 0.f;
     int cnt_local = 0;
     const size_t trow = ((size_t)f * NYX + c) * g.max_views;
-    for (int p = grp; p < nv; p += G) {
-      const int v = vlist[p];
-      int first, cnt;
-      float vals[XT_MAXA];
-      if constexpr (TAB) {
-        const float4 t4 = B.at_vals[trow + p];
-        first = B.at_first[trow + p];
-        vals[0] = t4.x;
-        vals[1] = t4.y;
-        vals[2] = t4.z;
-        vals[3] = t4.w;
+    for (int p0 = grp; p0 < nv; p0 += 2 * G) {
+      int first[2], cnt[2];
+      float vals[2][XT_MAXA];
 #pragma unroll
-        for (int b = 4; b < XT_MAXA; b++) vals[b] = 0.f;
-        cnt = (t4.x > 0.f) + (t4.y > 0.f) + (t4.z > 0.f) + (t4.w > 0.f);
-      } else {
-        cnt = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx,
-                                &first, vals);
-      }
-      if (zl == 0) {
-        a_first[p] = first;
+      for (int u = 0; u < 2; u++) {
+        const int p = p0 + u * G;
+        first[u] = 0;
+        cnt[u] = 0;
 #pragma unroll
-        for (int b = 0; b < XT_MAXA; b++) a_vals[(size_t)p * XT_MAXA + b] = vals[b];
-        cnt_local += cnt;
-      }
-      if (zok) {
-        const size_t base = ((size_t)v * g.N_r + first) * g.N_t + zl;
-        float ev[XT_MAXA], wv[XT_MAXA];
-#pragma unroll
-        for (int b = 0; b < NA; b++) {
-          ev[b] = 0.f;
-          wv[b] = 0.f;
-          if (vals[b] > 0.f) {
-            ev[b] = B.e[base + (size_t)b * g.N_t];
-            wv[b] = B.w[base + (size_t)b * g.N_t];
-          }
-        }
-#pragma unroll
-        for (int b = 0; b < NA; b++) {
-          const float pe = vals[b] * g.delta_t;
-          // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
-          if (ev[b] * ev[b] * wv[b] < g.T * g.T) {
-            th2 += pe * pe * wv[b];
-            th1 += -(ev[b] * pe * wv[b]);
+        for (int b = 0; b < XT_MAXA; b++) vals[u][b] = 0.f;
+        if (p < nv) {
+          if constexpr (TAB) {
+            const float4 t4 = B.at_vals[trow + p];
+            first[u] = B.at_first[trow + p];
+            vals[u][0] = t4.x;
+            vals[u][1] = t4.y;
+            vals[u][2] = t4.z;
+            vals[u][3] = t4.w;
+            cnt[u] = (t4.x > 0.f) + (t4.y > 0.f) + (t4.z > 0.f) + (t4.w > 0.f);
           } else {
-            const float ts = g.T * g.delta * sqrtf(wv[b]) / fabsf(ev[b]);
-            th2 += pe * pe * ts;
-            th1 += -(ev[b] * pe * ts);
+            const int v = vlist[p];
+            cnt[u] = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v],
+                                       yy, xx, &first[u], vals[u]);
+          }
+          if (zl == 0) {
+            a_first[p] = first[u];
+            a_v4[p] = make_float4(vals[u][0], vals[u][1], vals[u][2], vals[u][3]);
+            if constexpr (!TAB) a_v2[p] = make_float2(vals[u][4], vals[u][5]);
+            cnt_local += cnt[u];
           }
         }
+      }
+      if (lane_on) {
+        float ev[2][NA], wv[2][NA];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int p = p0 + u * G;
+          const int v = p < nv ? vlist[p] : 0;
+          const size_t base = ((size_t)v * g.N_r + first[u]) * g.N_t + zl;
+#pragma unroll
+          for (int b = 0; b < NA; b++) {
+            ev[u][b] = 0.f;
+            wv[u][b] = 0.f;
+            if (vals[u][b] > 0.f) {
+              ev[u][b] = B.e[base + (size_t)b * g.N_t];
+              wv[u][b] = B.w[base + (size_t)b * g.N_t];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+          for (int b = 0; b < NA; b++) {
+            const float pe = vals[u][b] * g.delta_t;
+            const float e1 = ev[u][b], w1 = wv[u][b];
+            // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
+            if (e1 * e1 * w1 < g.T * g.T) {
+              th2 += pe * pe * w1;
+              th1 += -(e1 * pe * w1);
+            } else {
+              const float ts = g.T * g.delta * sqrtf(w1) / fabsf(e1);
+              th2 += pe * pe * ts;
+              th1 += -(e1 * pe * ts);
+            }
+          }
       }
     }
     red1[tid] = th1;
@@ -507,7 +538,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     const int n_steps = L.zorder ? g.N_z : 2;
     for (int step = 0; step < n_steps; step++) {
       const bool mine = L.zorder ? (zl == step) : ((zl & 1) == step);
-      if (tid < ZL && zok && mine) {
+      if (tid < ZL && lane_on && mine) {
         const int z = zl;
         const float V = xcol[z];
         float nb[27];
@@ -538,7 +569,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
         if (tf1) tn1 = B.x[((size_t)(f + 1) * NYX + c) * g.N_z + z];
         any_pos |= (tf0 && tn0 > 0.f) || (tf1 && tn1 > 0.f);
         // zero skipping, F/XT_VoxUpdate.c:208-232
-        const bool skip = (V == 0.f) && (L.iter > 1) && !any_pos;
+        const bool skip = (V == 0.f) && (iter > 1) && !any_pos;
         if (!skip) {
           float s1 = 0.f, s2 = 0.f;
 #pragma unroll
@@ -568,10 +599,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
           if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
           xcol[z] = nv2;
           delta[z] = nv2 - V;
-          const int blk = g.N_z / g.nb;
-          int bidx = blk > 0 ? z / blk : 0;
-          if (bidx >= g.nb) bidx = g.nb - 1;
-          atomicAdd(&magb[bidx], fabsf(nv2 - V));
+          atomicAdd(&magb[zblk], fabsf(nv2 - V));
           vox_mag += fabsf(nv2);
           atomicAdd(&s_perf, 1);
         }
@@ -581,13 +609,14 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
 
     // ---- write the line back and its update magnitude
     if (tid < ZL && zok) B.x[((size_t)f * NYX + c) * g.N_z + zl] = xcol[zl];
-    if (tid < g.nb) B.mag_blk[((size_t)f * g.nb + tid) * NYX + c] = magb[tid];
-    if (tid == 0) {
-      float s = 0.f;
-      for (int b = 0; b < g.nb; b++) s += magb[b];
-      B.magsum[(size_t)f * NYX + c] = s;
-      n_upd += (double)s_perf;
-      n_foot += (double)s_perf * (double)s_cnt_col * (double)g.zo;
+    if (tid < g.nb) {
+      // MagUpdateMap of this line: reset and re-accumulated by the blocks that
+      // updated it (F/XT_VoxUpdate.c:128,236); other blocks keep theirs
+      const size_t mi = ((size_t)f * g.nb + tid) * NYX + c;
+      if (L.mode == 0 || B.nhicd_blk[mi] != 0)
+        B.mag_blk[mi] = magb[tid];
+      else
+        magb[tid] = B.mag_blk[mi];
     }
 
     // ---- e -= A * delta over the same footprint
@@ -597,9 +626,16 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
         for (int p = grp; p < nv; p += G) {
           const int v = vlist[p];
           const size_t base = ((size_t)v * g.N_r + a_first[p]) * g.N_t + zl;
+          const float4 a4 = a_v4[p];
+          float av[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, 0.f, 0.f};
+          if constexpr (!TAB) {
+            const float2 a2 = a_v2[p];
+            av[4] = a2.x;
+            av[5] = a2.y;
+          }
 #pragma unroll
           for (int b = 0; b < NA; b++) {
-            const float a = a_vals[(size_t)p * XT_MAXA + b];
+            const float a = av[b];
             if (a > 0.f) {
               const float pe = a * g.delta_t;
               __hip_atomic_fetch_add(&B.e[base + (size_t)b * g.N_t], -(pe * dz), __ATOMIC_RELAXED,
@@ -611,6 +647,11 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     }
     __syncthreads();
     if (tid == 0) {
+      float sm = 0.f;
+      for (int b = 0; b < g.nb; b++) sm += magb[b];
+      B.magsum[(size_t)f * NYX + c] = sm;
+      n_upd += (double)s_perf;
+      n_foot += (double)s_perf * (double)s_cnt_col * (double)g.zo;
       s_cnt_col = 0;
       s_perf = 0;
     }
@@ -936,6 +977,17 @@ __global__ void __launch_bounds__(1024) k_vsc_select(int NYX, const float *__res
   if (threadIdx.x == 0) n_out[m] = cnt;
 }
 
+// A line is an NHICD line when any of its z-blocks selected it.
+__global__ void k_vsc_any(int nb, size_t NYX, int n_time, const uint8_t *__restrict__ blk,
+                          uint8_t *__restrict__ any) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NYX * n_time) return;
+  const size_t f = i / NYX, c = i % NYX;
+  uint8_t a = 0;
+  for (int b = 0; b < nb; b++) a |= blk[(f * nb + b) * NYX + c];
+  any[i] = a;
+}
+
 // ---------------------------------------------------------------------------
 // Layout changes between the host's [N_time][N_z(+2)][N_y][N_x] object and the
 // device's [N_time][N_y][N_x][N_z].
@@ -974,4 +1026,12 @@ __global__ void k_select(XtGeom g, const float *__restrict__ e, const float *__r
 __global__ void k_zero_f64(double *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// Start of an iteration inside a captured graph: advance the iteration number
+// and clear the per-iteration statistics.
+__global__ void k_begin_iteration(int *iter, double *stats, int n) {
+  const int i = threadIdx.x;
+  if (i == 0) *iter += 1;
+  if (i < n) stats[i] = 0.0;
 }

@@ -51,19 +51,19 @@ static int set_err(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0};
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0};
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 9) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 10) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 9) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 10) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -173,8 +173,9 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
 }
 
 static size_t icd_lds_bytes(const XtGeom &g, int ZL) {
-  return (size_t)g.max_views * (XT_MAXA * sizeof(float) + sizeof(int)) + 2 * 256 * sizeof(float) +
-         2 * ZL * sizeof(float) + 64 * sizeof(float) + 256 * sizeof(int);
+  // per view: float4 + float2 (A column) + int (first bin)
+  return (size_t)g.max_views * (sizeof(float4) + sizeof(float2) + sizeof(int)) +
+         2 * 256 * sizeof(float) + 2 * ZL * sizeof(float) + 64 * sizeof(float) + 256 * sizeof(int);
 }
 
 template <typename T>
@@ -213,13 +214,17 @@ struct xt_icd_state {
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
   int *tiles_by_color = nullptr, *n_tiles = nullptr, *tile_start = nullptr, *tile_cols = nullptr;
   int max_tiles = 0, n_tiles_tot = 0;
-  uint8_t *nhicd_flag = nullptr;
+  uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
   float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr;
   int *nsel = nullptr, *nsel_out = nullptr;
   double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
   double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
   float4 *at_vals = nullptr;  // A-column table (null: columns built on the fly)
   int *at_first = nullptr;
+  int *iter_dev = nullptr;      // iteration number read by the voxel-update kernel
+  hipGraph_t graph = nullptr;   // one captured iteration (iterations >= 2)
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_launches = 0;   // voxel-update launches in the captured iteration
   int off_nch = 1, off_chunk = 1, off_nblk = 1;
   // statistics
   double total_updates = 0, total_foot = 0, icd_ms = 0, last_cost = NAN, stop_pct = NAN;
@@ -234,9 +239,12 @@ static void free_state(xt_icd_state *s) {
   void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
                   s->tiles_by_color, s->n_tiles, s->tile_start, s->tile_cols, s->nhicd_flag,
+                  s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
-                  s->at_vals,  s->at_first};
+                  s->at_vals,  s->at_first,   s->iter_dev};
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -301,11 +309,13 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   B.tile_start = s->tile_start;
   B.tile_cols = s->tile_cols;
   B.nhicd_flag = s->nhicd_flag;
+  B.nhicd_blk = s->nhicd_blk;
   B.mag_blk = s->mag_blk;
   B.magsum = s->magsum;
   B.stats = s->stats;
   B.at_vals = s->at_vals;
   B.at_first = s->at_first;
+  B.iter = s->iter_dev;
   return B;
 }
 
@@ -371,11 +381,16 @@ static int variance_update(xt_icd_state *s) {
   return XT_OK;
 }
 
+// NHICD selection per (frame, z-block) map as F/XT_NHICD.c:178-205, then the
+// per-line union that decides which lines the NHICD launches visit.
 static int vsc_select(xt_icd_state *s) {
-  dim3 gf((unsigned)((s->NYX + 255) / 256), s->g.N_time);
-  hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->magsum, s->filt);
-  hipLaunchKernelGGL(k_vsc_select, dim3(s->g.N_time), dim3(1024), 0, s->st, (int)s->NYX, s->filt,
-                     s->nsel, s->nhicd_flag, s->nsel_out);
+  const int M = s->g.N_time * s->g.nb;
+  dim3 gf((unsigned)((s->NYX + 255) / 256), M);
+  hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->mag_blk, s->filt);
+  hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
+                     s->nhicd_blk, s->nsel_out);
+  hipLaunchKernelGGL(k_vsc_any, dim3((unsigned)((s->NYX * s->g.N_time + 255) / 256)), dim3(256), 0,
+                     s->st, s->g.nb, s->NYX, s->g.N_time, s->nhicd_blk, s->nhicd_flag);
   XT_HIP(hipGetLastError());
   return XT_OK;
 }
@@ -602,13 +617,17 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       (rc = dev_alloc(&s->y, s->nsino)) || (rc = dev_alloc(&s->d, (size_t)g.N_r * g.N_t)) ||
       (rc = dev_alloc(&s->x, s->nobj)) || (rc = dev_alloc(&s->Hr, (size_t)g.N_p * XT_HR_STRIDE)) ||
       (rc = dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX)) ||
+      (rc = dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX)) ||
-      (rc = dev_alloc(&s->filt, (size_t)g.N_time * s->NYX)) ||
-      (rc = dev_alloc(&s->nsel, g.N_time)) || (rc = dev_alloc(&s->nsel_out, g.N_time)) ||
+      (rc = dev_alloc(&s->filt, (size_t)g.N_time * g.nb * s->NYX)) ||
+      (rc = dev_alloc(&s->nsel, g.N_time * g.nb)) ||
+      (rc = dev_alloc(&s->nsel_out, g.N_time * g.nb)) ||
       (rc = dev_alloc(&s->stats, 8)) || (rc = dev_alloc(&s->red_wg, 8192)) ||
-      (rc = dev_alloc(&s->scal, 8)))
+      (rc = dev_alloc(&s->scal, 8)) || (rc = dev_alloc(&s->iter_dev, 1)))
     return fail(rc);
+  if (hipMemset(s->iter_dev, 0, sizeof(int)) != hipSuccess)
+    return fail(set_err(XT_E_HIP, "iteration counter clear failed"));
   // ring-offset scratch: view chunks so that N_r*N_t*chunks >= ~256k threads
   {
     const int NRT = g.N_r * g.N_t;
@@ -632,11 +651,13 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     return fail(set_err(XT_E_HIP, "sinogram upload failed"));
   if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
       hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)) != hipSuccess ||
-      hipMemset(s->nhicd_flag, 0, (size_t)g.N_time * s->NYX) != hipSuccess)
+      hipMemset(s->nhicd_flag, 0, (size_t)g.N_time * s->NYX) != hipSuccess ||
+      hipMemset(s->nhicd_blk, 0, (size_t)g.N_time * g.nb * s->NYX) != hipSuccess)
     return fail(set_err(XT_E_HIP, "scratch clear failed"));
   {
-    // NHICD lines per frame: floor(UpdateSelectNum / NHICD_Iterations), F/XT_NHICD.c:182
-    std::vector<int> ns(g.N_time, s->n_masked / s->K);
+    // NHICD lines per (frame, z-block): floor(UpdateSelectNum / NHICD_Iterations),
+    // F/XT_NHICD.c:182 (every block holds all masked lines)
+    std::vector<int> ns((size_t)g.N_time * g.nb, s->n_masked / s->K);
     if (hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(XT_E_HIP, "nsel upload"));
   }
@@ -690,15 +711,14 @@ extern "C" int xt_icd_cost(xt_icd_state *s, double *cost) {
   return XT_OK;
 }
 
-static int iterate_one(xt_icd_state *s) {
-  const int Iter = ++s->iter;
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s->st, s->stats, 8);
+// The kernels of one iteration, enqueued on s->st (eagerly, or into a graph
+// under capture): the schedule of F/XT_ICD_update.c:794-882.
+static int enqueue_iteration(xt_icd_state *s, bool nhicd) {
+  hipLaunchKernelGGL(k_begin_iteration, dim3(1), dim3(64), 0, s->st, s->iter_dev, s->stats, 8);
   XtIcdLaunch L;
   L.zorder = 0;
   L.K = s->K;
-  L.iter = Iter;
   L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
-  const bool nhicd = Iter > 1 && !s->no_nhicd;
   for (int j = 0; j < s->K; j++) {
     L.sub_iter = j;
     L.mode = 0;
@@ -718,6 +738,37 @@ static int iterate_one(xt_icd_state *s) {
         if (s->update_offset > 1 && (ph & 1)) XT_TRY(offset_update(s));
       }
     }
+  }
+  return XT_OK;
+}
+
+static int iterate_one(xt_icd_state *s) {
+  const int Iter = ++s->iter;
+  const bool nhicd = Iter > 1 && !s->no_nhicd;
+  // iterations from the second on are all the same launch sequence: capture it
+  // once and replay it (no memset nodes; the iteration number lives on the device)
+  const bool use_graph = g_opt[XT_OPT_GRAPH] != 0 && Iter > 1 && !s->timing;
+  if (use_graph) {
+    if (!s->graph_exec) {
+      XT_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+      const int64_t l0 = s->launches;
+      int rc = enqueue_iteration(s, nhicd);
+      s->graph_launches = s->launches - l0;
+      s->launches = l0;
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(s->st, &gr);
+      if (rc != XT_OK) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      if (e != hipSuccess) return set_err(XT_E_HIP, "graph capture: %s", hipGetErrorString(e));
+      s->graph = gr;
+      XT_HIP(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+    }
+    XT_HIP(hipGraphLaunch(s->graph_exec, s->st));
+    s->launches += s->graph_launches;
+  } else {
+    XT_TRY(enqueue_iteration(s, nhicd));
   }
   if (s->update_var == 1) XT_TRY(variance_update(s));
   // stop statistic, F/XT_ICD_update.c:887-925
@@ -1222,17 +1273,20 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
   XT_TRY(dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX));
   XT_HIP(hipMemset(s->nhicd_flag, 1, (size_t)g.N_time * s->NYX));
+  XT_TRY(dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX));
+  XT_HIP(hipMemset(s->nhicd_blk, 1, (size_t)g.N_time * g.nb * s->NYX));
   XT_TRY(dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX));
   XT_TRY(dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX));
   XT_HIP(hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)));
   XT_TRY(dev_alloc(&s->stats, 8));
   XT_HIP(hipMemset(s->stats, 0, 8 * sizeof(double)));
+  XT_TRY(dev_alloc(&s->iter_dev, 1));
+  XT_HIP(hipMemcpy(s->iter_dev, &Iter, sizeof(int), hipMemcpyHostToDevice));
   XtIcdLaunch L;
   L.mode = 1;
   L.phase = (4 - (frame & 3)) & 3;
   L.sub_iter = 0;
   L.K = 1;
-  L.iter = Iter;
   L.seed = 0;
   L.count_stats = 1;
   L.zorder = 1;
