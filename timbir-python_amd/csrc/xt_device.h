@@ -265,8 +265,10 @@ __device__ __forceinline__ float xt_qggmrf_ratio(float delta, float p, float c, 
 
 // Per-launch work description of the voxel-update kernel.
 struct XtIcdLaunch {
-  int mode;        // 0 homogeneous sub-iteration, 1 NHICD lines
-  int phase;       // 0..3: frame f updates tiles of colour (phase + f) & 3
+  int mode;        // 0 homogeneous share, 1 NHICD lines, 2 the given list in order (one
+                   // workgroup; per-voxel checks)
+  int half;        // 0/1: first or second half of the share (the reference's even and
+                   // odd z-block passes, each followed by a ring-offset update)
   int sub_iter;    // j of K
   int K;           // NHICD_Iterations
   uint32_t seed;
@@ -283,11 +285,10 @@ struct XtIcdBuffers {
   const float *cosv, *sinv;
   const int *views;          // [N_time][max_views]
   const int *proj_num;       // [N_time]
-  const int *tiles_by_color; // [4][max_tiles]
-  const int *n_tiles;        // [4]
-  int max_tiles;
-  const int *tile_start;     // [n_tiles_total + 1] into tile_cols
-  const int *tile_cols;      // packed y*N_x + x, masked columns in tile raster order
+  const int *lines;          // masked lines y*N_x + x [n_lines] (mode 2: the list to update)
+  int n_lines;
+  const int *nhicd_list;     // [N_time][N_y*N_x]: NHICD lines of each frame (raster order)
+  const int *nhicd_count;    // [N_time]
   const uint8_t *nhicd_flag; // [N_time][N_y][N_x]: line selected by some z-block
   const uint8_t *nhicd_blk;  // [N_time][nb][N_y][N_x]: selected by that z-block
   float *mag_blk;            // [N_time][nb][N_y][N_x]
@@ -348,45 +349,71 @@ __device__ __forceinline__ uint32_t xt_hash(uint32_t a) {
   return a;
 }
 
+// A keyed pseudo-random permutation of [0, n): a 4-round Feistel network on
+// the smallest even number of bits covering n, with cycle walking back into
+// range (each walk ends: the cycle through i < n returns below n).
+__device__ __forceinline__ int xt_permute(int i, int n, uint32_t key) {
+  int bits = 2;
+  while ((1 << bits) < n) bits += 2;
+  const int h = bits >> 1;
+  const uint32_t mask = (1u << h) - 1u;
+  uint32_t v = (uint32_t)i;
+  do {
+    uint32_t l = v >> h, r = v & mask;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t t = l ^ (xt_hash(r ^ (key + 0x9E3779B9u * (uint32_t)(k + 1))) & mask);
+      l = r;
+      r = t;
+    }
+    v = (l << h) | r;
+  } while (v >= (uint32_t)n);
+  return (int)v;
+}
+
 // ---------------------------------------------------------------------------
-// Voxel-line update of one super-voxel tile per workgroup (F/XT_VoxUpdate.c:
-// 11-254 with the surrogate of F/XT_ICD_update.c:130-179).  The workgroup walks
-// its tile's lines one after another.  For each line, threads (view group grp,
-// slice zl) sweep the sinogram footprint for theta1/theta2 (the A column of
-// each view is built once and kept in LDS), wave 0 updates the slices in
-// red-black order (exact Gauss-Seidel: slices of one line share no sinogram
-// entries), and all threads subtract A*delta from e with no-return atomics.
-// Tiles of one colour are never adjacent, so concurrently updated lines are
-// not prior neighbours; the colour is shifted per frame so adjacent frames
-// never update the same line at once.
+// Voxel-line updates (F/XT_VoxUpdate.c:11-254 with the surrogate of
+// F/XT_ICD_update.c:130-179).  Grid = (workgroups per frame, frames); each
+// workgroup takes lines i = lo + blockIdx.x + k*gridDim.x of the launch's range
+// of a random order of the frame's lines (the reference's random_select order
+// and its xy_start..xy_end share, F/XT_ICD_update.c:599-631,807-809) and
+// updates them one after another.  Per line:
+//  1. all threads stage the line's A columns (table or built on the fly) and
+//     the 3x3 neighbourhood columns plus the temporal neighbours into LDS;
+//  2. threads (view group grp, slice zl) sweep the sinogram footprint for
+//     theta1/theta2, U views in flight per step, the view walk rotated per line
+//     so concurrent workgroups spread over the whole sinogram;
+//  3. wave 0 updates the slices in red-black order (exact Gauss-Seidel: slices
+//     of a line share no sinogram entries);
+//  4. all threads subtract A*delta from e with no-return atomics, and wait for
+//     them before the next line so its loads (which bypass L1) see them.
+// Lines updated concurrently by different workgroups read each other's e and
+// x as they happen to be (a bounded Jacobi coupling; the concurrency is capped
+// on the host).
 template <int ZL, bool TAB>
-__global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+__global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
   constexpr int NT = 256;
   constexpr int G = NT / ZL;
+  constexpr int U = TAB ? 4 : 2;          // views in flight per group and step
+  constexpr int NA = TAB ? 4 : XT_MAXA;   // bins a column can hold
   extern __shared__ float4 smem4[];
   float4 *a_v4 = smem4;                                            // [max_views] bins 0..3
   float2 *a_v2 = (float2 *)(a_v4 + g.max_views);                   // [max_views] bins 4..5
-  int *a_first = (int *)(a_v2 + g.max_views);                      // [max_views]
-  float *red1 = (float *)(a_first + g.max_views);                  // [NT]
+  int *a_base = (int *)(a_v2 + (TAB ? 0 : g.max_views));           // [max_views] e offset
+  float *red1 = (float *)(a_base + g.max_views);                   // [NT]
   float *red2 = red1 + NT;                                         // [NT]
-  float *xcol = red2 + NT;                                         // [ZL]
-  float *delta = xcol + ZL;                                        // [ZL]
+  float *xn = red2 + NT;                                           // [11][ZL]
+  float *delta = xn + 11 * ZL;                                     // [ZL]
   float *magb = delta + ZL;                                        // [64]
-  int *order = (int *)(magb + 64);                                 // [256]
-  __shared__ int s_n_lines, s_cnt_col, s_perf;
+  __shared__ int s_cnt_col, s_perf;
 
   const int tid = threadIdx.x;
   const int zl = tid % ZL, grp = tid / ZL;
   const int f = blockIdx.y;
-  const int color = (L.phase + f) & 3;
-  if ((int)blockIdx.x >= B.n_tiles[color]) return;
-  const int tile = B.tiles_by_color[color * B.max_tiles + blockIdx.x];
-  const int t0 = B.tile_start[tile], nt = B.tile_start[tile + 1] - t0;
   const int nv = B.proj_num[f];
   const int *vlist = B.views + (size_t)f * g.max_views;
   const size_t NYX = (size_t)g.N_y * g.N_x;
   const bool zok = zl < g.N_z;
-  constexpr int NA = TAB ? 4 : XT_MAXA;  // bins a column can hold
   const int iter = *B.iter;
   // z-block of this slice: block b holds slices [b*floor(N_z/nb), (b+1)*floor(N_z/nb)),
   // the last one up to N_z-1 (F/XT_ICD_update.c:804-806), so with fewer slices
@@ -395,112 +422,162 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
   int zblk = blk_sz > 0 ? zl / blk_sz : g.nb - 1;
   if (zblk >= g.nb) zblk = g.nb - 1;
 
-  // ---- lines of the tile this launch updates, in order
+  // ---- this launch's range of positions, and the order that maps them to lines
+  int lo = 0, hi = 0, n_dom = 0;
+  const int *list = B.lines;
+  uint32_t key = 0;
+  if (L.mode == 0) {
+    const int n = B.n_lines;
+    const int per = n / L.K;
+    const int xs = L.sub_iter * per;
+    const int xe = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
+    const int mid = xs + (xe - xs) / 2;
+    lo = L.half ? mid : xs;
+    hi = L.half ? xe : mid;
+    n_dom = n;
+    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+  } else if (L.mode == 1) {
+    const int n = B.nhicd_count[f];
+    lo = L.half ? n / 2 : 0;
+    hi = L.half ? n : n / 2;
+    n_dom = n;
+    list = B.nhicd_list + (size_t)f * NYX;
+    key = xt_hash(L.seed ^ 0x5BD1E995u ^
+                  xt_hash((uint32_t)iter * 0xC2B2AE35u ^
+                          xt_hash((uint32_t)(f * 131 + L.sub_iter) + 29u)));
+  } else {
+    lo = 0;
+    hi = B.n_lines;
+  }
   if (tid == 0) {
-    int n = 0;
-    if (L.mode == 0) {
-      // a random order of the tile's lines for this iteration (the same in every
-      // sub-iteration), then its j-th 1/K part as F/XT_ICD_update.c:807-809
-      for (int i = 0; i < nt; i++) order[i] = B.tile_cols[t0 + i];
-      uint32_t h = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x9E3779B9U ^
-                                            xt_hash((uint32_t)f * 7919U + (uint32_t)tile)));
-      for (int i = nt - 1; i > 0; i--) {
-        h = xt_hash(h + 0x6D2B79F5U);
-        const int jx = (int)(h % (uint32_t)(i + 1));
-        const int tmp = order[i];
-        order[i] = order[jx];
-        order[jx] = tmp;
-      }
-      const int per = nt / L.K;
-      const int xs = L.sub_iter * per;
-      const int xe = (L.sub_iter == L.K - 1) ? nt - 1 : (L.sub_iter + 1) * per - 1;
-      for (int i = xs; i <= xe; i++) order[n++] = order[i];
-    } else {
-      for (int i = 0; i < nt; i++) {
-        const int c = B.tile_cols[t0 + i];
-        if (B.nhicd_flag[(size_t)f * NYX + c]) order[n++] = c;
-      }
-    }
-    s_n_lines = n;
     s_cnt_col = 0;
     s_perf = 0;
   }
-  __syncthreads();
-  const int n_lines = s_n_lines;
   float vox_mag = 0.f;     // lane-local sum of |new x| (wave 0)
   double n_upd = 0.0, n_foot = 0.0;
+  const size_t tbl = (size_t)f * NYX * g.max_views;
 
-  for (int li = 0; li < n_lines; li++) {
-    const int c = order[li];
+  for (int i = lo + (int)blockIdx.x; i < hi; i += (int)gridDim.x) {
+    const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
     // NHICD lines: a slice is updated only if its own z-block selected the line
     // (F/XT_ICD_update.c:844-881 walk each block's own list)
     const bool lane_on =
-        zok && (L.mode == 0 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
-    // earlier lines' e and x writes of this workgroup must be seen by its loads
-    __threadfence();
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        zok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
 
-    // ---- theta pass over the footprint, two views per step so each wave keeps
-    // the loads of both in flight
-    float th1 = 0.f, th2 = 0.f;
+    // ---- 1. stage the A columns and the neighbourhood
     int cnt_local = 0;
-    const size_t trow = ((size_t)f * NYX + c) * g.max_views;
-    for (int p0 = grp; p0 < nv; p0 += 2 * G) {
-      int first[2], cnt[2];
-      float vals[2][XT_MAXA];
+    if constexpr (TAB) {
+      // four table entries per thread in flight at once
+      const size_t row = tbl + (size_t)c * g.max_views;
+      for (int p0 = tid; p0 < nv; p0 += 4 * NT) {
+        float4 t4[4];
+        int first[4], v[4];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const int p = p0 + u * G;
-        first[u] = 0;
-        cnt[u] = 0;
-#pragma unroll
-        for (int b = 0; b < XT_MAXA; b++) vals[u][b] = 0.f;
-        if (p < nv) {
-          if constexpr (TAB) {
-            const float4 t4 = B.at_vals[trow + p];
-            first[u] = B.at_first[trow + p];
-            vals[u][0] = t4.x;
-            vals[u][1] = t4.y;
-            vals[u][2] = t4.z;
-            vals[u][3] = t4.w;
-            cnt[u] = (t4.x > 0.f) + (t4.y > 0.f) + (t4.z > 0.f) + (t4.w > 0.f);
-          } else {
-            const int v = vlist[p];
-            cnt[u] = xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v],
-                                       yy, xx, &first[u], vals[u]);
+        for (int u = 0; u < 4; u++) {
+          const int p = p0 + u * NT;
+          t4[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+          first[u] = 0;
+          v[u] = 0;
+          if (p < nv) {
+            t4[u] = B.at_vals[row + p];
+            first[u] = B.at_first[row + p];
+            v[u] = vlist[p];
           }
-          if (zl == 0) {
-            a_first[p] = first[u];
-            a_v4[p] = make_float4(vals[u][0], vals[u][1], vals[u][2], vals[u][3]);
-            if constexpr (!TAB) a_v2[p] = make_float2(vals[u][4], vals[u][5]);
-            cnt_local += cnt[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int p = p0 + u * NT;
+          if (p < nv) {
+            a_v4[p] = t4[u];
+            a_base[p] = (v[u] * g.N_r + first[u]) * g.N_t;
+            cnt_local += (t4[u].x > 0.f) + (t4[u].y > 0.f) + (t4[u].z > 0.f) + (t4[u].w > 0.f);
+          }
+        }
+      }
+    } else {
+      for (int p = tid; p < nv; p += NT) {
+        const int v = vlist[p];
+        int first = 0;
+        float vals[XT_MAXA];
+        xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx, &first,
+                          vals);
+        a_v4[p] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+        a_v2[p] = make_float2(vals[4], vals[5]);
+        a_base[p] = (v * g.N_r + first) * g.N_t;
+#pragma unroll
+        for (int b = 0; b < XT_MAXA; b++) cnt_local += vals[b] > 0.f;
+      }
+    }
+    for (int q = tid; q < 11 * ZL; q += NT) {
+      const int k = q / ZL, z = q - k * ZL;
+      float val = 0.f;
+      if (z < g.N_z) {
+        if (k < 9) {
+          const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+          if (y2 >= 0 && y2 < g.N_y && x2 >= 0 && x2 < g.N_x)
+            val = __hip_atomic_load(&B.x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z],
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          const int f2 = f + (k == 9 ? -1 : 1);
+          if (f2 >= 0 && f2 < g.N_time)
+            val = __hip_atomic_load(&B.x[((size_t)f2 * NYX + c) * g.N_z + z], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      xn[q] = val;
+    }
+    if (cnt_local) atomicAdd(&s_cnt_col, cnt_local);
+    __syncthreads();
+
+    // ---- 2. theta pass
+    const int rot = (int)(xt_hash((uint32_t)c * 0x27D4EB2Fu ^ key ^ (uint32_t)L.half) %
+                          (uint32_t)(nv > 0 ? nv : 1));
+    float th1 = 0.f, th2 = 0.f;
+    for (int q0 = grp; q0 < nv; q0 += U * G) {
+      int base[U];
+      float av[U][NA];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = q0 + u * G;
+        base[u] = 0;
+#pragma unroll
+        for (int b = 0; b < NA; b++) av[u][b] = 0.f;
+        if (q < nv) {
+          int p = q + rot;
+          if (p >= nv) p -= nv;
+          const float4 a4 = a_v4[p];
+          base[u] = a_base[p];
+          av[u][0] = a4.x;
+          av[u][1] = a4.y;
+          av[u][2] = a4.z;
+          av[u][3] = a4.w;
+          if constexpr (!TAB) {
+            const float2 a2 = a_v2[p];
+            av[u][4] = a2.x;
+            av[u][5] = a2.y;
           }
         }
       }
       if (lane_on) {
-        float ev[2][NA], wv[2][NA];
+        float ev[U][NA], wv[U][NA];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const int p = p0 + u * G;
-          const int v = p < nv ? vlist[p] : 0;
-          const size_t base = ((size_t)v * g.N_r + first[u]) * g.N_t + zl;
+        for (int u = 0; u < U; u++)
 #pragma unroll
           for (int b = 0; b < NA; b++) {
             ev[u][b] = 0.f;
             wv[u][b] = 0.f;
-            if (vals[u][b] > 0.f) {
-              ev[u][b] = B.e[base + (size_t)b * g.N_t];
-              wv[u][b] = B.w[base + (size_t)b * g.N_t];
+            if (av[u][b] > 0.f) {
+              const int idx = base[u] + b * g.N_t + zl;
+              ev[u][b] = __hip_atomic_load(&B.e[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              wv[u][b] = B.w[idx];
             }
           }
-        }
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < U; u++)
 #pragma unroll
           for (int b = 0; b < NA; b++) {
-            const float pe = vals[u][b] * g.delta_t;
+            const float pe = av[u][b] * g.delta_t;
             const float e1 = ev[u][b], w1 = wv[u][b];
             // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
             if (e1 * e1 * w1 < g.T * g.T) {
@@ -516,10 +593,9 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     }
     red1[tid] = th1;
     red2[tid] = th2;
-    if (zl == 0 && cnt_local) atomicAdd(&s_cnt_col, cnt_local);
     __syncthreads();
 
-    // ---- theta reduction and the line's current values (wave 0)
+    // ---- theta reduction (wave 0)
     if (tid < ZL) {
       float t1 = 0.f, t2 = 0.f;
       for (int q = 0; q < G; q++) {
@@ -528,13 +604,14 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
       }
       red1[tid] = t1;
       red2[tid] = t2;
-      xcol[tid] = zok ? B.x[((size_t)f * NYX + c) * g.N_z + zl] : 0.f;
       delta[tid] = 0.f;
     }
     if (tid < 64) magb[tid] = 0.f;
     __syncthreads();
 
-    // ---- surrogate update of every slice, red-black in z (or ascending)
+    // ---- 3. surrogate update of every slice, red-black in z (or ascending);
+    // the line's own column is xn[4][*] and is updated in place
+    float *xcol = xn + 4 * ZL;
     const int n_steps = L.zorder ? g.N_z : 2;
     for (int step = 0; step < n_steps; step++) {
       const bool mine = L.zorder ? (zl == step) : ((zl & 1) == step);
@@ -554,19 +631,13 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
               const int z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
               const bool ok = (l != 13) && z2 >= 0 && z2 < g.N_z && y2 >= 0 && y2 < g.N_y &&
                               x2 >= 0 && x2 < g.N_x;
-              float val = 0.f;
-              if (ok)
-                val = (dy == 0 && dx == 0)
-                          ? xcol[z2]
-                          : B.x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z2];
+              const float val = ok ? xn[((dy + 1) * 3 + (dx + 1)) * ZL + z2] : 0.f;
               nb[l] = val;
               fl[l] = ok;
               any_pos |= ok && (val > 0.f);
             }
-        float tn0 = 0.f, tn1 = 0.f;
         const bool tf0 = f > 0, tf1 = f + 1 < g.N_time;
-        if (tf0) tn0 = B.x[((size_t)(f - 1) * NYX + c) * g.N_z + z];
-        if (tf1) tn1 = B.x[((size_t)(f + 1) * NYX + c) * g.N_z + z];
+        const float tn0 = xn[9 * ZL + z], tn1 = xn[10 * ZL + z];
         any_pos |= (tf0 && tn0 > 0.f) || (tf1 && tn1 > 0.f);
         // zero skipping, F/XT_VoxUpdate.c:208-232
         const bool skip = (V == 0.f) && (iter > 1) && !any_pos;
@@ -608,24 +679,27 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     }
 
     // ---- write the line back and its update magnitude
-    if (tid < ZL && zok) B.x[((size_t)f * NYX + c) * g.N_z + zl] = xcol[zl];
+    if (tid < ZL && zok)
+      __hip_atomic_store(&B.x[((size_t)f * NYX + c) * g.N_z + zl], xcol[zl], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     if (tid < g.nb) {
       // MagUpdateMap of this line: reset and re-accumulated by the blocks that
       // updated it (F/XT_VoxUpdate.c:128,236); other blocks keep theirs
       const size_t mi = ((size_t)f * g.nb + tid) * NYX + c;
-      if (L.mode == 0 || B.nhicd_blk[mi] != 0)
+      if (L.mode != 1 || B.nhicd_blk[mi] != 0)
         B.mag_blk[mi] = magb[tid];
       else
         magb[tid] = B.mag_blk[mi];
     }
 
-    // ---- e -= A * delta over the same footprint
+    // ---- 4. e -= A * delta over the same footprint
     if (zok) {
       const float dz = delta[zl];
       if (dz != 0.f) {
-        for (int p = grp; p < nv; p += G) {
-          const int v = vlist[p];
-          const size_t base = ((size_t)v * g.N_r + a_first[p]) * g.N_t + zl;
+        for (int q = grp; q < nv; q += G) {
+          int p = q + rot;
+          if (p >= nv) p -= nv;
+          const int base = a_base[p];
           const float4 a4 = a_v4[p];
           float av[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, 0.f, 0.f};
           if constexpr (!TAB) {
@@ -638,13 +712,16 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
             const float a = av[b];
             if (a > 0.f) {
               const float pe = a * g.delta_t;
-              __hip_atomic_fetch_add(&B.e[base + (size_t)b * g.N_t], -(pe * dz), __ATOMIC_RELAXED,
+              __hip_atomic_fetch_add(&B.e[base + b * g.N_t + zl], -(pe * dz), __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
             }
           }
         }
       }
     }
+    // the next line's loads must see these atomics: wait for all of this
+    // thread's memory operations, then for every thread of the workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
       float sm = 0.f;
@@ -655,6 +732,7 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
       s_cnt_col = 0;
       s_perf = 0;
     }
+    __syncthreads();
   }
   // ---- per-workgroup statistics
   if (tid < ZL && L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
@@ -662,6 +740,40 @@ __global__ void __launch_bounds__(256) k_icd_tiles(XtGeom g, XtIcdBuffers B, XtI
     atomicAdd(&B.stats[1], n_upd);
     atomicAdd(&B.stats[2], n_foot);
   }
+}
+
+// NHICD line list of each frame: the selected lines inside the mask, in raster
+// order, and their number.  One workgroup of 1024 threads per frame.
+__global__ void __launch_bounds__(1024) k_nhicd_compact(int NYX, const uint8_t *__restrict__ flag,
+                                                        const uint8_t *__restrict__ mask,
+                                                        int *__restrict__ list,
+                                                        int *__restrict__ count) {
+  __shared__ int wsum[16];
+  __shared__ int s_base;
+  const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint8_t *fl = flag + (size_t)f * NYX;
+  int *out = list + (size_t)f * NYX;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < NYX; c0 += 1024) {
+    const int c = c0 + tid;
+    const bool on = c < NYX && fl[c] && mask[c];
+    const unsigned long long bal = __ballot(on);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wid] = __popcll(bal);
+    __syncthreads();
+    int off = s_base;
+    for (int k = 0; k < wid; k++) off += wsum[k];
+    if (on) out[off + before] = c;
+    __syncthreads();
+    if (tid == 0) {
+      int t = 0;
+      for (int k = 0; k < 16; k++) t += wsum[k];
+      s_base += t;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) count[f] = s_base;
 }
 
 // ---------------------------------------------------------------------------

@@ -2,12 +2,13 @@
 // and seam entry points of include/xt_icd.h.
 //
 // One process, one GPU, one stream.  Per ICD iteration the host enqueues, for
-// each of K = NHICD_Iterations sub-iterations: four colour launches of the
-// voxel-update kernel over 1/K of every tile's lines, the ring-offset update
-// after each half, the NHICD selection, and four colour launches over the
-// selected lines (from the second iteration on) -- the schedule of
-// Source_Code_4D_Fast/XT_ICD_update.c:767-926 with z-blocks replaced by whole
-// voxel lines and OpenMP threads replaced by super-voxel tiles.
+// each of K = NHICD_Iterations sub-iterations: two launches of the voxel-update
+// kernel over the two halves of this sub-iteration's 1/K share of a random
+// order of every frame's lines, a ring-offset update after each, the NHICD
+// selection, and two launches over the selected lines (from the second
+// iteration on) -- the schedule of Source_Code_4D_Fast/XT_ICD_update.c:767-926
+// with z-blocks replaced by whole voxel lines and OpenMP threads by up to 1024
+// workgroups per frame, each updating one line at a time.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -172,10 +173,11 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
   return XT_OK;
 }
 
-static size_t icd_lds_bytes(const XtGeom &g, int ZL) {
-  // per view: float4 + float2 (A column) + int (first bin)
-  return (size_t)g.max_views * (sizeof(float4) + sizeof(float2) + sizeof(int)) +
-         2 * 256 * sizeof(float) + 2 * ZL * sizeof(float) + 64 * sizeof(float) + 256 * sizeof(int);
+// dynamic LDS of k_icd_lines: per view the A column (float4, plus float2 for
+// bins 4..5 without the table) and its e offset; theta partials; neighbourhood.
+static size_t icd_lds_bytes(const XtGeom &g, int ZL, bool table) {
+  return (size_t)g.max_views * (sizeof(float4) + (table ? 0 : sizeof(float2)) + sizeof(int)) +
+         2 * 256 * sizeof(float) + 12 * ZL * sizeof(float) + 64 * sizeof(float);
 }
 
 template <typename T>
@@ -205,15 +207,15 @@ struct xt_icd_state {
   int K = 10, NumIter = 0, update_offset = 0, zero_mean = 0, update_var = 0, no_nhicd = 0;
   float var_est = 1.f, stop_threshold = 0.f;
   int iter = 0, stopped = 0;
-  int TS = 8, n_masked = 0;
+  int n_masked = 0, conc = 16;  // masked lines; workgroups (concurrent lines) per frame
   size_t nsino = 0, nobj = 0, NYX = 0;
   TomoInputs *T = nullptr;  // for debug lines only
   // device buffers
   float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *x = nullptr, *Hr = nullptr;
   float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr;
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
-  int *tiles_by_color = nullptr, *n_tiles = nullptr, *tile_start = nullptr, *tile_cols = nullptr;
-  int max_tiles = 0, n_tiles_tot = 0;
+  int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
+  uint8_t *line_mask = nullptr;
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
   float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr;
   int *nsel = nullptr, *nsel_out = nullptr;
@@ -238,7 +240,7 @@ static void free_state(xt_icd_state *s) {
   if (!s) return;
   void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
-                  s->tiles_by_color, s->n_tiles, s->tile_start, s->tile_cols, s->nhicd_flag,
+                  s->lines,    s->nhicd_list, s->nhicd_count, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
@@ -286,10 +288,10 @@ template <int ZL>
 static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
                        int gx, size_t lds) {
   if (B.at_vals)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL, true>), dim3(gx, g.N_time), dim3(256), lds,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true>), dim3(gx, g.N_time), dim3(256), lds,
                        st, g, B, L);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_tiles<ZL, false>), dim3(gx, g.N_time), dim3(256), lds,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false>), dim3(gx, g.N_time), dim3(256), lds,
                        st, g, B, L);
 }
 
@@ -303,11 +305,10 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   B.sinv = s->sinv;
   B.views = s->views;
   B.proj_num = s->proj_num;
-  B.tiles_by_color = s->tiles_by_color;
-  B.n_tiles = s->n_tiles;
-  B.max_tiles = s->max_tiles;
-  B.tile_start = s->tile_start;
-  B.tile_cols = s->tile_cols;
+  B.lines = s->lines;
+  B.n_lines = s->n_masked;
+  B.nhicd_list = s->nhicd_list;
+  B.nhicd_count = s->nhicd_count;
   B.nhicd_flag = s->nhicd_flag;
   B.nhicd_blk = s->nhicd_blk;
   B.mag_blk = s->mag_blk;
@@ -319,9 +320,11 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   return B;
 }
 
-static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L) {
+// One voxel-update launch over a range of lines; gx workgroups per frame.
+static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L, int gx) {
   XtIcdBuffers B = buffers_of(s);
-  size_t lds = icd_lds_bytes(s->g, s->ZL);
+  size_t lds = icd_lds_bytes(s->g, s->ZL, B.at_vals != nullptr);
+  if (gx < 1) gx = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing) {
     if (s->ev_used + 2 > (int)s->ev.size()) {
@@ -337,10 +340,10 @@ static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L) {
     XT_HIP(hipEventRecord(e0, s->st));
   }
   switch (s->ZL) {
-    case 8: icd_launch<8>(s->st, s->g, B, L, s->max_tiles, lds); break;
-    case 16: icd_launch<16>(s->st, s->g, B, L, s->max_tiles, lds); break;
-    case 32: icd_launch<32>(s->st, s->g, B, L, s->max_tiles, lds); break;
-    default: icd_launch<64>(s->st, s->g, B, L, s->max_tiles, lds); break;
+    case 8: icd_launch<8>(s->st, s->g, B, L, gx, lds); break;
+    case 16: icd_launch<16>(s->st, s->g, B, L, gx, lds); break;
+    case 32: icd_launch<32>(s->st, s->g, B, L, gx, lds); break;
+    default: icd_launch<64>(s->st, s->g, B, L, gx, lds); break;
   }
   XT_HIP(hipGetLastError());
   if (s->timing) XT_HIP(hipEventRecord(e1, s->st));
@@ -391,65 +394,36 @@ static int vsc_select(xt_icd_state *s) {
                      s->nhicd_blk, s->nsel_out);
   hipLaunchKernelGGL(k_vsc_any, dim3((unsigned)((s->NYX * s->g.N_time + 255) / 256)), dim3(256), 0,
                      s->st, s->g.nb, s->NYX, s->g.N_time, s->nhicd_blk, s->nhicd_flag);
+  hipLaunchKernelGGL(k_nhicd_compact, dim3(s->g.N_time), dim3(1024), 0, s->st, (int)s->NYX,
+                     s->nhicd_flag, s->line_mask, s->nhicd_list, s->nhicd_count);
   XT_HIP(hipGetLastError());
   return XT_OK;
 }
 
-static int build_tiles(xt_icd_state *s, const std::vector<int> &masked_cols) {
-  const int NY = s->g.N_y, NX = s->g.N_x;
-  int TS = (int)g_opt[XT_OPT_TILE];
-  if (TS <= 0) {
-    // concurrent tiles per colour: at most 1024 and at most 1/64 of the lines
-    int ncol = NY * NX;
-    int target = ncol / 64;
-    if (target > 1024) target = 1024;
-    if (target < 16) target = 16;
-    TS = 1;
-    while (TS < 16) {
-      int per_color = ((NY + TS - 1) / TS) * ((NX + TS - 1) / TS) / 4;
-      if (per_color <= target) break;
-      TS *= 2;
-    }
+// The masked lines on the device, and how many lines of a frame are updated
+// at once: at most 1024 and at most 1/64 of the lines (concurrently updated
+// lines read each other's stale error sinogram; a CPU simulation of that
+// coupling kept the reference's trajectory up to ~6% of the lines at once
+// and diverged at 25%), or XT_OPT_LINES when set.
+static int setup_lines(xt_icd_state *s, const std::vector<int> &masked) {
+  int conc = (int)g_opt[XT_OPT_LINES];
+  if (conc <= 0) {
+    conc = s->n_masked / 64;
+    if (conc > 1024) conc = 1024;
+    if (conc < 16) conc = 16;
   }
-  if (TS < 1 || TS > 16) return set_err(XT_E_ARG, "tile side %d outside 1..16", TS);
-  s->TS = TS;
-  const int TY = (NY + TS - 1) / TS, TX = (NX + TS - 1) / TS;
-  std::vector<std::vector<int>> cols(TY * TX);
-  std::vector<uint8_t> in(NY * NX, 0);
-  for (int c : masked_cols) in[c] = 1;
-  for (int yy = 0; yy < NY; yy++)
-    for (int xx = 0; xx < NX; xx++)
-      if (in[yy * NX + xx]) cols[(yy / TS) * TX + (xx / TS)].push_back(yy * NX + xx);
-  std::vector<int> start(1, 0), flat, bycolor[4];
-  int nt = 0;
-  std::vector<int> tile_id(TY * TX, -1);
-  for (int ty = 0; ty < TY; ty++)
-    for (int tx = 0; tx < TX; tx++) {
-      auto &v = cols[ty * TX + tx];
-      if (v.empty()) continue;
-      flat.insert(flat.end(), v.begin(), v.end());
-      start.push_back((int)flat.size());
-      bycolor[(ty & 1) * 2 + (tx & 1)].push_back(nt);
-      nt++;
-    }
-  int mt = 1;
-  for (int c = 0; c < 4; c++) mt = (int)bycolor[c].size() > mt ? (int)bycolor[c].size() : mt;
-  std::vector<int> tbc(4 * mt, 0), ntc(4);
-  for (int c = 0; c < 4; c++) {
-    ntc[c] = (int)bycolor[c].size();
-    for (size_t i = 0; i < bycolor[c].size(); i++) tbc[c * mt + i] = bycolor[c][i];
-  }
-  s->max_tiles = mt;
-  s->n_tiles_tot = nt;
-  XT_TRY(dev_alloc(&s->tiles_by_color, tbc.size()));
-  XT_TRY(dev_alloc(&s->n_tiles, 4));
-  XT_TRY(dev_alloc(&s->tile_start, start.size()));
-  XT_TRY(dev_alloc(&s->tile_cols, flat.size()));
-  XT_HIP(hipMemcpy(s->tiles_by_color, tbc.data(), tbc.size() * sizeof(int), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->n_tiles, ntc.data(), 4 * sizeof(int), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->tile_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!flat.empty())
-    XT_HIP(hipMemcpy(s->tile_cols, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (conc > 65535) return set_err(XT_E_ARG, "XT_OPT_LINES %d > 65535", conc);
+  s->conc = conc;
+  std::vector<uint8_t> in(s->NYX, 0);
+  for (int c : masked) in[c] = 1;
+  XT_TRY(dev_alloc(&s->lines, masked.empty() ? 1 : masked.size()));
+  XT_TRY(dev_alloc(&s->line_mask, s->NYX));
+  XT_TRY(dev_alloc(&s->nhicd_list, (size_t)s->g.N_time * s->NYX));
+  XT_TRY(dev_alloc(&s->nhicd_count, s->g.N_time));
+  if (!masked.empty())
+    XT_HIP(hipMemcpy(s->lines, masked.data(), masked.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->line_mask, in.data(), s->NYX, hipMemcpyHostToDevice));
+  XT_HIP(hipMemset(s->nhicd_count, 0, s->g.N_time * sizeof(int)));
   return XT_OK;
 }
 
@@ -596,11 +570,13 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   s->no_nhicd = T->no_NHICD;
   s->var_est = T->var_est;
   s->stop_threshold = T->StopThreshold;
-  if (icd_lds_bytes(g, s->ZL) > 64 * 1024)
+  if (icd_lds_bytes(g, s->ZL, false) > 64 * 1024)
     return fail(set_err(XT_E_ARG, "%d views in one frame exceed the LDS footprint cache",
                         g.max_views));
   s->nsino = (size_t)g.N_p * g.N_r * g.N_t;
   s->NYX = (size_t)g.N_y * g.N_x;
+  if (s->nsino >= ((size_t)1 << 31))
+    return fail(set_err(XT_E_ARG, "sinogram of %zu entries: at most 2^31 - 1", s->nsino));
   s->nobj = (size_t)g.N_time * s->NYX * g.N_z;
   // masked lines, F/XT_ICD_update.c:958-974
   std::vector<int> masked;
@@ -611,7 +587,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       if (xc * xc + yc * yc < T->radius_obj * T->radius_obj) masked.push_back(j * g.N_x + k);
     }
   s->n_masked = (int)masked.size();
-  if ((rc = build_tiles(s, masked))) return fail(rc);
+  if ((rc = setup_lines(s, masked))) return fail(rc);
   if ((rc = upload_views(s, S, O))) return fail(rc);
   if ((rc = dev_alloc(&s->e, s->nsino)) || (rc = dev_alloc(&s->w, s->nsino)) ||
       (rc = dev_alloc(&s->y, s->nsino)) || (rc = dev_alloc(&s->d, (size_t)g.N_r * g.N_t)) ||
@@ -719,23 +695,29 @@ static int enqueue_iteration(xt_icd_state *s, bool nhicd) {
   L.zorder = 0;
   L.K = s->K;
   L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
+  // per sub-iteration: the two halves of the homogeneous share, then the two
+  // halves of the NHICD lines, each followed by a ring-offset update -- the
+  // even/odd z-block passes of F/XT_ICD_update.c:799-881
+  const int per = s->n_masked / s->K;
   for (int j = 0; j < s->K; j++) {
     L.sub_iter = j;
     L.mode = 0;
     L.count_stats = 1;
-    for (int ph = 0; ph < 4; ph++) {
-      L.phase = ph;
-      XT_TRY(launch_icd(s, L));
-      if (s->update_offset > 1 && (ph & 1)) XT_TRY(offset_update(s));
+    const int share = (j == s->K - 1) ? s->n_masked - j * per : per;
+    for (int h = 0; h < 2; h++) {
+      L.half = h;
+      const int n = h ? share - share / 2 : share / 2;
+      XT_TRY(launch_icd(s, L, n < s->conc ? n : s->conc));
+      if (s->update_offset > 1) XT_TRY(offset_update(s));
     }
     if (nhicd) {
       XT_TRY(vsc_select(s));
       L.mode = 1;
       L.count_stats = 0;
-      for (int ph = 0; ph < 4; ph++) {
-        L.phase = ph;
-        XT_TRY(launch_icd(s, L));
-        if (s->update_offset > 1 && (ph & 1)) XT_TRY(offset_update(s));
+      for (int h = 0; h < 2; h++) {
+        L.half = h;
+        XT_TRY(launch_icd(s, L, s->conc));
+        if (s->update_offset > 1) XT_TRY(offset_update(s));
       }
     }
   }
@@ -891,8 +873,8 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   }
   xt_icd_state *s = nullptr;
   XT_TRY(create_impl(S, O, T, &s));
-  dbg(T, "ICD_BackProject: %s, tiles %dx%d (%d), z lanes %d, A table %s\n", xt_version(), s->TS,
-      s->TS, s->n_tiles_tot, s->ZL, s->at_vals ? "on" : "off");
+  dbg(T, "ICD_BackProject: %s, %d masked lines, %d updated at once per frame, z lanes %d, "
+      "A table %s\n", xt_version(), s->n_masked, s->conc, s->ZL, s->at_vals ? "on" : "off");
   // the error sinogram now lives on the GPU; Projection is released as the
   // reference does (F/XT_ICD_update.c:999)
   xt_multifree(S->Projection, 3);
@@ -1242,26 +1224,18 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(seam_state(c, S, O, T));
   xt_icd_state *s = c.s;
   const XtGeom &g = s->g;
-  if (icd_lds_bytes(g, s->ZL) > 64 * 1024) return set_err(XT_E_ARG, "too many views");
+  if (icd_lds_bytes(g, s->ZL, false) > 64 * 1024) return set_err(XT_E_ARG, "too many views");
   std::vector<int> cols(n_lines);
   for (int i = 0; i < n_lines; i++) {
     if (x_list[i] < 0 || x_list[i] >= g.N_x || y_list[i] < 0 || y_list[i] >= g.N_y)
       return set_err(XT_E_ARG, "line %d out of range", i);
     cols[i] = y_list[i] * g.N_x + x_list[i];
   }
-  // one tile of colour 0 holding the lines in list order, updated with every
-  // line "selected" (NHICD mode walks the tile list in order)
-  int start[2] = {0, n_lines}, tbc[4] = {0, 0, 0, 0}, ntc[4] = {1, 0, 0, 0};
-  s->max_tiles = 1;
-  XT_TRY(dev_alloc(&s->tiles_by_color, 4));
-  XT_TRY(dev_alloc(&s->n_tiles, 4));
-  XT_TRY(dev_alloc(&s->tile_start, 2));
-  XT_TRY(dev_alloc(&s->tile_cols, n_lines ? n_lines : 1));
-  XT_HIP(hipMemcpy(s->tiles_by_color, tbc, sizeof tbc, hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->n_tiles, ntc, sizeof ntc, hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->tile_start, start, sizeof start, hipMemcpyHostToDevice));
+  // the lines in list order, one workgroup (mode 2)
+  s->n_masked = n_lines;
+  XT_TRY(dev_alloc(&s->lines, n_lines ? n_lines : 1));
   if (n_lines)
-    XT_HIP(hipMemcpy(s->tile_cols, cols.data(), n_lines * sizeof(int), hipMemcpyHostToDevice));
+    XT_HIP(hipMemcpy(s->lines, cols.data(), n_lines * sizeof(int), hipMemcpyHostToDevice));
   XT_TRY(upload_views(s, S, O));
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
   XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
@@ -1283,14 +1257,14 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(dev_alloc(&s->iter_dev, 1));
   XT_HIP(hipMemcpy(s->iter_dev, &Iter, sizeof(int), hipMemcpyHostToDevice));
   XtIcdLaunch L;
-  L.mode = 1;
-  L.phase = (4 - (frame & 3)) & 3;
+  L.mode = 2;
+  L.half = 0;
   L.sub_iter = 0;
   L.K = 1;
   L.seed = 0;
   L.count_stats = 1;
   L.zorder = 1;
-  XT_TRY(launch_icd(s, L));
+  XT_TRY(launch_icd(s, L, 1));
   XT_HIP(hipStreamSynchronize(s->st));
   XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
   XT_TRY(download_object(s, O));

@@ -18,7 +18,7 @@ from . import xt_abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
-(OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_TILE, OPT_DEVICE, OPT_VERBOSE,
+(OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH) = range(1, 11)
 
 # every symbol include/xt_icd.h declares
