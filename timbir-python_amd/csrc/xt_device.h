@@ -267,8 +267,9 @@ __device__ __forceinline__ float xt_qggmrf_ratio(float delta, float p, float c, 
 struct XtIcdLaunch {
   int mode;        // 0 homogeneous share, 1 NHICD lines, 2 the given list in order (one
                    // workgroup; per-voxel checks)
-  int half;        // 0/1: first or second half of the share (the reference's even and
-                   // odd z-block passes, each followed by a ring-offset update)
+  int half;        // 0/1: the reference's even and odd z-block passes: frame f updates
+                   // the slices of z-blocks with parity half ^ (f & 1)
+                   // (F/XT_ICD_update.c:799-881); each is followed by a ring-offset update
   int sub_iter;    // j of K
   int K;           // NHICD_Iterations
   uint32_t seed;
@@ -287,9 +288,11 @@ struct XtIcdBuffers {
   const int *proj_num;       // [N_time]
   const int *lines;          // masked lines y*N_x + x [n_lines] (mode 2: the list to update)
   int n_lines;
-  const int *nhicd_list;     // [N_time][N_y*N_x]: NHICD lines of each frame (raster order)
-  const int *nhicd_count;    // [N_time]
-  const uint8_t *nhicd_flag; // [N_time][N_y][N_x]: line selected by some z-block
+  const int *nhicd_list;     // [N_time][2][N_y*N_x]: NHICD lines of each frame and z-block
+                             // parity (raster order)
+  const int *nhicd_count;    // [N_time][2]
+  const uint8_t *nhicd_flag; // [N_time][2][N_y][N_x]: line selected by some z-block of
+                             // that parity
   const uint8_t *nhicd_blk;  // [N_time][nb][N_y][N_x]: selected by that z-block
   float *mag_blk;            // [N_time][nb][N_y][N_x]
   float *magsum;             // [N_time][N_y][N_x]
@@ -426,25 +429,20 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   int lo = 0, hi = 0, n_dom = 0;
   const int *list = B.lines;
   uint32_t key = 0;
+  // z-block parity this launch updates (every block for the list mode)
+  const int par = (L.mode == 2) ? -1 : ((L.half ^ f) & 1);
   if (L.mode == 0) {
     const int n = B.n_lines;
     const int per = n / L.K;
-    const int xs = L.sub_iter * per;
-    const int xe = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
-    const int mid = xs + (xe - xs) / 2;
-    lo = L.half ? mid : xs;
-    hi = L.half ? xe : mid;
+    lo = L.sub_iter * per;
+    hi = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
     n_dom = n;
     key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
   } else if (L.mode == 1) {
-    const int n = B.nhicd_count[f];
-    lo = L.half ? n / 2 : 0;
-    hi = L.half ? n : n / 2;
-    n_dom = n;
-    list = B.nhicd_list + (size_t)f * NYX;
-    key = xt_hash(L.seed ^ 0x5BD1E995u ^
-                  xt_hash((uint32_t)iter * 0xC2B2AE35u ^
-                          xt_hash((uint32_t)(f * 131 + L.sub_iter) + 29u)));
+    lo = 0;
+    hi = B.nhicd_count[f * 2 + par];
+    list = B.nhicd_list + ((size_t)f * 2 + par) * NYX;
+    key = xt_hash(L.seed ^ 0x5BD1E995u ^ xt_hash((uint32_t)iter * 0xC2B2AE35u ^ (uint32_t)f));
   } else {
     lo = 0;
     hi = B.n_lines;
@@ -457,13 +455,24 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   double n_upd = 0.0, n_foot = 0.0;
   const size_t tbl = (size_t)f * NYX * g.max_views;
 
-  for (int i = lo + (int)blockIdx.x; i < hi; i += (int)gridDim.x) {
+  // homogeneous lines: positions blockIdx.x, +gridDim.x, ... of the random order;
+  // NHICD lines: a contiguous run of the raster-ordered list per workgroup (the
+  // reference walks its selection lists in raster order, F/XT_NHICD.c:189-199)
+  int i0 = lo + (int)blockIdx.x, i1 = hi, istep = (int)gridDim.x;
+  if (L.mode == 1) {
+    const long long len = hi - lo;
+    i0 = lo + (int)(len * blockIdx.x / gridDim.x);
+    i1 = lo + (int)(len * (blockIdx.x + 1) / gridDim.x);
+    istep = 1;
+  }
+  for (int i = i0; i < i1; i += istep) {
     const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
     // NHICD lines: a slice is updated only if its own z-block selected the line
     // (F/XT_ICD_update.c:844-881 walk each block's own list)
     const bool lane_on =
-        zok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
+        zok && (par < 0 || (zblk & 1) == par) &&
+        (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
 
     // ---- 1. stage the A columns and the neighbourhood
     int cnt_local = 0;
@@ -686,7 +695,7 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
       // MagUpdateMap of this line: reset and re-accumulated by the blocks that
       // updated it (F/XT_VoxUpdate.c:128,236); other blocks keep theirs
       const size_t mi = ((size_t)f * g.nb + tid) * NYX + c;
-      if (L.mode != 1 || B.nhicd_blk[mi] != 0)
+      if ((par < 0 || (tid & 1) == par) && (L.mode != 1 || B.nhicd_blk[mi] != 0))
         B.mag_blk[mi] = magb[tid];
       else
         magb[tid] = B.mag_blk[mi];
@@ -742,8 +751,9 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   }
 }
 
-// NHICD line list of each frame: the selected lines inside the mask, in raster
-// order, and their number.  One workgroup of 1024 threads per frame.
+// NHICD line list of each (frame, parity) map: the selected lines inside the
+// mask, in raster order, and their number.  One workgroup of 1024 threads per
+// map.
 __global__ void __launch_bounds__(1024) k_nhicd_compact(int NYX, const uint8_t *__restrict__ flag,
                                                         const uint8_t *__restrict__ mask,
                                                         int *__restrict__ list,
@@ -1089,15 +1099,17 @@ __global__ void __launch_bounds__(1024) k_vsc_select(int NYX, const float *__res
   if (threadIdx.x == 0) n_out[m] = cnt;
 }
 
-// A line is an NHICD line when any of its z-blocks selected it.
+// A line is an NHICD line of a z-block parity when a z-block of that parity
+// selected it: any[f][par][c].
 __global__ void k_vsc_any(int nb, size_t NYX, int n_time, const uint8_t *__restrict__ blk,
                           uint8_t *__restrict__ any) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= NYX * n_time) return;
   const size_t f = i / NYX, c = i % NYX;
-  uint8_t a = 0;
-  for (int b = 0; b < nb; b++) a |= blk[(f * nb + b) * NYX + c];
-  any[i] = a;
+  uint8_t a[2] = {0, 0};
+  for (int b = 0; b < nb; b++) a[b & 1] |= blk[(f * nb + b) * NYX + c];
+  any[(f * 2 + 0) * NYX + c] = a[0];
+  any[(f * 2 + 1) * NYX + c] = a[1];
 }
 
 // ---------------------------------------------------------------------------

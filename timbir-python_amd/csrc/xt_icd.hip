@@ -394,7 +394,7 @@ static int vsc_select(xt_icd_state *s) {
                      s->nhicd_blk, s->nsel_out);
   hipLaunchKernelGGL(k_vsc_any, dim3((unsigned)((s->NYX * s->g.N_time + 255) / 256)), dim3(256), 0,
                      s->st, s->g.nb, s->NYX, s->g.N_time, s->nhicd_blk, s->nhicd_flag);
-  hipLaunchKernelGGL(k_nhicd_compact, dim3(s->g.N_time), dim3(1024), 0, s->st, (int)s->NYX,
+  hipLaunchKernelGGL(k_nhicd_compact, dim3(s->g.N_time * 2), dim3(1024), 0, s->st, (int)s->NYX,
                      s->nhicd_flag, s->line_mask, s->nhicd_list, s->nhicd_count);
   XT_HIP(hipGetLastError());
   return XT_OK;
@@ -418,12 +418,12 @@ static int setup_lines(xt_icd_state *s, const std::vector<int> &masked) {
   for (int c : masked) in[c] = 1;
   XT_TRY(dev_alloc(&s->lines, masked.empty() ? 1 : masked.size()));
   XT_TRY(dev_alloc(&s->line_mask, s->NYX));
-  XT_TRY(dev_alloc(&s->nhicd_list, (size_t)s->g.N_time * s->NYX));
-  XT_TRY(dev_alloc(&s->nhicd_count, s->g.N_time));
+  XT_TRY(dev_alloc(&s->nhicd_list, (size_t)s->g.N_time * 2 * s->NYX));
+  XT_TRY(dev_alloc(&s->nhicd_count, s->g.N_time * 2));
   if (!masked.empty())
     XT_HIP(hipMemcpy(s->lines, masked.data(), masked.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->line_mask, in.data(), s->NYX, hipMemcpyHostToDevice));
-  XT_HIP(hipMemset(s->nhicd_count, 0, s->g.N_time * sizeof(int)));
+  XT_HIP(hipMemset(s->nhicd_count, 0, s->g.N_time * 2 * sizeof(int)));
   return XT_OK;
 }
 
@@ -592,7 +592,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = dev_alloc(&s->e, s->nsino)) || (rc = dev_alloc(&s->w, s->nsino)) ||
       (rc = dev_alloc(&s->y, s->nsino)) || (rc = dev_alloc(&s->d, (size_t)g.N_r * g.N_t)) ||
       (rc = dev_alloc(&s->x, s->nobj)) || (rc = dev_alloc(&s->Hr, (size_t)g.N_p * XT_HR_STRIDE)) ||
-      (rc = dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX)) ||
+      (rc = dev_alloc(&s->nhicd_flag, (size_t)g.N_time * 2 * s->NYX)) ||
       (rc = dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX)) ||
@@ -627,7 +627,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     return fail(set_err(XT_E_HIP, "sinogram upload failed"));
   if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
       hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)) != hipSuccess ||
-      hipMemset(s->nhicd_flag, 0, (size_t)g.N_time * s->NYX) != hipSuccess ||
+      hipMemset(s->nhicd_flag, 0, (size_t)g.N_time * 2 * s->NYX) != hipSuccess ||
       hipMemset(s->nhicd_blk, 0, (size_t)g.N_time * g.nb * s->NYX) != hipSuccess)
     return fail(set_err(XT_E_HIP, "scratch clear failed"));
   {
@@ -695,9 +695,9 @@ static int enqueue_iteration(xt_icd_state *s, bool nhicd) {
   L.zorder = 0;
   L.K = s->K;
   L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
-  // per sub-iteration: the two halves of the homogeneous share, then the two
-  // halves of the NHICD lines, each followed by a ring-offset update -- the
-  // even/odd z-block passes of F/XT_ICD_update.c:799-881
+  // per sub-iteration: the even and the odd z-block pass over the homogeneous
+  // share, then over the NHICD lines, each followed by a ring-offset update
+  // (F/XT_ICD_update.c:799-881)
   const int per = s->n_masked / s->K;
   for (int j = 0; j < s->K; j++) {
     L.sub_iter = j;
@@ -706,8 +706,7 @@ static int enqueue_iteration(xt_icd_state *s, bool nhicd) {
     const int share = (j == s->K - 1) ? s->n_masked - j * per : per;
     for (int h = 0; h < 2; h++) {
       L.half = h;
-      const int n = h ? share - share / 2 : share / 2;
-      XT_TRY(launch_icd(s, L, n < s->conc ? n : s->conc));
+      XT_TRY(launch_icd(s, L, share < s->conc ? share : s->conc));
       if (s->update_offset > 1) XT_TRY(offset_update(s));
     }
     if (nhicd) {
