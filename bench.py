@@ -86,8 +86,9 @@ def gpu_timed(cfg, path, xt_hip, steps, warmup):
     foot = s1.footprint_entries - s0.footprint_entries
     kms = s1.icd_kernel_ms - s0.icd_kernel_ms
     nl = s1.icd_launches - s0.icd_launches
+    vis = s1.line_visits - s0.line_visits
     return dict(seconds=dt, updates=upd, footprint=foot, kernel_ms=kms, launches=nl,
-                iters=steps)
+                iters=steps, visits=vis)
 
 
 def gpu_converge(cfg, path, xt_hip, n_iter):
@@ -227,6 +228,7 @@ def main():
                      "launches_per_step": g["launches"] / a.steps,
                      "bytes_per_step": 12.0 * g["footprint"] / a.steps},
         "voxel_updates_per_step": g["updates"] / a.steps,
+        "line_visits_per_step": g["visits"] / a.steps,
         "time_to_converge": ({"seconds": conv["seconds"], "iterations": conv["iters"],
                               "criterion": f"MAP cost within 1e-4 of iteration {a.converge_iters}"}
                              if conv else None),

@@ -78,9 +78,16 @@ enum {
   XT_OPT_ATABLE = 9,         /* 1: keep every line's A columns in an HBM table built once
                                 (when it fits and each column spans <= 4 bins); 0: build
                                 them on every visit (default 1) */
-  XT_OPT_GRAPH = 10          /* 1: capture one iteration's launches as a hipGraph and replay
+  XT_OPT_GRAPH = 10,         /* 1: capture one iteration's launches as a hipGraph and replay
                                 it from the second iteration on (not while per-launch timing
                                 is on); 0: enqueue every launch (default 1) */
+  XT_OPT_PACK = 11,          /* 1: a pass over one z-block parity packs that parity's slices
+                                into half the lanes when the blocks are of equal size and
+                                even in number; 0: one lane per slice (default 1) */
+  XT_OPT_NHICD_LINES = 12    /* 0: NHICD selects per z-block as the reference does
+                                (F/XT_NHICD.c:178-205); 1: per z-block parity on the
+                                parity's summed update map, so a selected line is updated in
+                                every block of the parity (default 0) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);
@@ -105,6 +112,8 @@ typedef struct {
   double last_cost;             /* MAP cost after the last iteration (XT_OPT_COST) */
   double stop_percentage;       /* 100 * sum|dx| / sum|x|, the reference's stop test */
   int32_t stopped;
+  double line_visits;           /* workgroup visits of a voxel line (a visit updates the line's
+                                   slices of one z-block parity) */
 } xt_icd_stats;
 
 int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state);

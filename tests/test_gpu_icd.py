@@ -174,3 +174,31 @@ def test_bad_input_is_an_error(scan_root):
     os.remove(os.path.join(path, "weight_n0.bin"))
     with pytest.raises(xt_hip.XtError):
         xt_hip.engine_run(path, cfg.cli_args())
+
+
+def test_packed_lanes_match_one_lane_per_slice(scan_root):
+    """A pass over one z-block parity packs that parity's slices into half the
+    lanes; with one line at a time (a deterministic sequential order) it must
+    reproduce the one-lane-per-slice kernel to float rounding of the theta sums.
+    NHICD is off: its order-statistic threshold turns rounding-level differences
+    of the update maps into different line sets, and early iterations amplify
+    those."""
+    cfg = dataclasses.replace(TINY, iterations=3, no_nhicd=True)
+    path, info = make_scan(cfg, scan_root, "pack")
+    vols = {}
+    xt_hip.set_option(xt_hip.OPT_LINES, 1)
+    try:
+        for pack in (0, 1):
+            xt_hip.set_option(xt_hip.OPT_PACK, pack)
+            with xt_hip.Engine(path, cfg.cli_args()) as eng:
+                with xt_hip.Icd(eng) as icd:
+                    icd.iterate(3)
+                    icd.download()
+                vols[pack] = eng.volume().copy()
+    finally:
+        xt_hip.set_option(xt_hip.OPT_PACK, 1)
+        xt_hip.set_option(xt_hip.OPT_LINES, 0)
+    err = np.abs(vols[1] - vols[0]).max() / np.abs(vols[0]).max()
+    print(f"  packed vs one lane per slice: max rel diff {err:.2e}")
+    assert np.abs(vols[0]).max() > 0
+    assert err <= 1e-4

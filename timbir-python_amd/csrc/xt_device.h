@@ -296,7 +296,8 @@ struct XtIcdBuffers {
   const uint8_t *nhicd_blk;  // [N_time][nb][N_y][N_x]: selected by that z-block
   float *mag_blk;            // [N_time][nb][N_y][N_x]
   float *magsum;             // [N_time][N_y][N_x]
-  double *stats;             // [0] total_vox_mag, [1] voxel updates, [2] footprint entries
+  double *stats;             // [0] total_vox_mag, [1] voxel updates, [2] footprint entries,
+                             // [3] line visits
   const float4 *at_vals;     // A-column table [N_time][N_y*N_x][max_views] (or null)
   const int *at_first;       // first bin of each table entry
   const int *iter;           // ICD iteration number (device memory, so a captured
@@ -379,26 +380,33 @@ __device__ __forceinline__ int xt_permute(int i, int n, uint32_t key) {
 // F/XT_ICD_update.c:130-179).  Grid = (workgroups per frame, frames); each
 // workgroup takes lines i = lo + blockIdx.x + k*gridDim.x of the launch's range
 // of a random order of the frame's lines (the reference's random_select order
-// and its xy_start..xy_end share, F/XT_ICD_update.c:599-631,807-809) and
-// updates them one after another.  Per line:
+// and its xy_start..xy_end share, F/XT_ICD_update.c:599-631,807-809), or a
+// contiguous run of the raster-ordered NHICD list, and updates them one after
+// another, each visit updating the line's slices of one z-block parity.  Per
+// visit:
 //  1. all threads stage the line's A columns (table or built on the fly) and
 //     the 3x3 neighbourhood columns plus the temporal neighbours into LDS;
-//  2. threads (view group grp, slice zl) sweep the sinogram footprint for
+//  2. threads (view group grp, lane zl) sweep the sinogram footprint for
 //     theta1/theta2, U views in flight per step, the view walk rotated per line
-//     so concurrent workgroups spread over the whole sinogram;
+//     so concurrent workgroups spread over the whole sinogram.  With PACK the
+//     active parity's slices are packed into ZL/2 lanes per view group (twice
+//     the views per step); buffer loads take the bin offset as a scalar, and a
+//     bin outside the column reads 0 through an out-of-range offset;
 //  3. wave 0 updates the slices in red-black order (exact Gauss-Seidel: slices
 //     of a line share no sinogram entries);
-//  4. all threads subtract A*delta from e with no-return atomics, and wait for
-//     them before the next line so its loads (which bypass L1) see them.
+//  4. all threads subtract A*delta from e with no-return buffer atomics, and
+//     wait for them before the next line, whose loads then see them.
 // Lines updated concurrently by different workgroups read each other's e and
 // x as they happen to be (a bounded Jacobi coupling; the concurrency is capped
 // on the host).
-template <int ZL, bool TAB>
+template <int ZL, bool TAB, bool PACK>
 __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
   constexpr int NT = 256;
-  constexpr int G = NT / ZL;
+  constexpr int ZP = PACK ? ZL / 2 : ZL;  // lanes per view group
+  constexpr int G = NT / ZP;              // view groups
   constexpr int U = TAB ? 4 : 2;          // views in flight per group and step
   constexpr int NA = TAB ? 4 : XT_MAXA;   // bins a column can hold
+  constexpr uint32_t OOB = 0x80000000u;   // buffer offset that reads 0 / drops the atomic
   extern __shared__ float4 smem4[];
   float4 *a_v4 = smem4;                                            // [max_views] bins 0..3
   float2 *a_v2 = (float2 *)(a_v4 + g.max_views);                   // [max_views] bins 4..5
@@ -407,30 +415,38 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   float *red2 = red1 + NT;                                         // [NT]
   float *xn = red2 + NT;                                           // [11][ZL]
   float *delta = xn + 11 * ZL;                                     // [ZL]
-  float *magb = delta + ZL;                                        // [64]
+  float *th1z = delta + ZL;                                        // [ZL]
+  float *th2z = th1z + ZL;                                         // [ZL]
+  float *magb = th2z + ZL;                                         // [64]
   __shared__ int s_cnt_col, s_perf;
 
   const int tid = threadIdx.x;
-  const int zl = tid % ZL, grp = tid / ZL;
   const int f = blockIdx.y;
   const int nv = B.proj_num[f];
   const int *vlist = B.views + (size_t)f * g.max_views;
   const size_t NYX = (size_t)g.N_y * g.N_x;
-  const bool zok = zl < g.N_z;
   const int iter = *B.iter;
-  // z-block of this slice: block b holds slices [b*floor(N_z/nb), (b+1)*floor(N_z/nb)),
+  const int blk_sz = g.N_z / g.nb;
+  // z-block parity this launch updates (every block for the list mode)
+  const int par = (L.mode == 2) ? -1 : ((L.half ^ f) & 1);
+  // this thread's lane: view group grp, slice zs (packed: the k-th slice of
+  // the active parity's blocks, blocks being blk_sz slices each)
+  const int grp = tid / ZP, lz = tid % ZP;
+  int zs = lz;
+  if constexpr (PACK) zs = ((lz / blk_sz) * 2 + par) * blk_sz + lz % blk_sz;
+  // z-block of a slice: block b holds slices [b*floor(N_z/nb), (b+1)*floor(N_z/nb)),
   // the last one up to N_z-1 (F/XT_ICD_update.c:804-806), so with fewer slices
   // than blocks every slice is in the last block
-  const int blk_sz = g.N_z / g.nb;
-  int zblk = blk_sz > 0 ? zl / blk_sz : g.nb - 1;
-  if (zblk >= g.nb) zblk = g.nb - 1;
+  auto zblock = [&](int z) {
+    int b = blk_sz > 0 ? z / blk_sz : g.nb - 1;
+    return b >= g.nb ? g.nb - 1 : b;
+  };
+  const bool zs_ok = zs < g.N_z && (par < 0 || (zblock(zs) & 1) == par);
 
   // ---- this launch's range of positions, and the order that maps them to lines
   int lo = 0, hi = 0, n_dom = 0;
   const int *list = B.lines;
   uint32_t key = 0;
-  // z-block parity this launch updates (every block for the list mode)
-  const int par = (L.mode == 2) ? -1 : ((L.half ^ f) & 1);
   if (L.mode == 0) {
     const int n = B.n_lines;
     const int per = n / L.K;
@@ -452,8 +468,13 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     s_perf = 0;
   }
   float vox_mag = 0.f;     // lane-local sum of |new x| (wave 0)
-  double n_upd = 0.0, n_foot = 0.0;
+  double n_upd = 0.0, n_foot = 0.0, n_vis = 0.0;
   const size_t tbl = (size_t)f * NYX * g.max_views;
+  const uint32_t sino_bytes = (uint32_t)((size_t)g.N_p * g.N_r * g.N_t * sizeof(float));
+  const __amdgpu_buffer_rsrc_t rs_e = __builtin_amdgcn_make_buffer_rsrc(B.e, 0, sino_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_w =
+      __builtin_amdgcn_make_buffer_rsrc((void *)B.w, 0, sino_bytes, 0x00020000);
+  const int bin_bytes = g.N_t * (int)sizeof(float);
 
   // homogeneous lines: positions blockIdx.x, +gridDim.x, ... of the random order;
   // NHICD lines: a contiguous run of the raster-ordered list per workgroup (the
@@ -468,13 +489,13 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   for (int i = i0; i < i1; i += istep) {
     const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
-    // NHICD lines: a slice is updated only if its own z-block selected the line
-    // (F/XT_ICD_update.c:844-881 walk each block's own list)
+    // lane of this thread active: its slice is in the launch's parity and, for
+    // NHICD lines, its own z-block selected the line (F/XT_ICD_update.c:844-881
+    // walk each block's own list)
     const bool lane_on =
-        zok && (par < 0 || (zblk & 1) == par) &&
-        (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblk) * NYX + c] != 0);
+        zs_ok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblock(zs)) * NYX + c] != 0);
 
-    // ---- 1. stage the A columns and the neighbourhood
+    // ---- 1. stage the A columns (scaled by delta_t) and the neighbourhood
     int cnt_local = 0;
     if constexpr (TAB) {
       // four table entries per thread in flight at once
@@ -498,7 +519,8 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
         for (int u = 0; u < 4; u++) {
           const int p = p0 + u * NT;
           if (p < nv) {
-            a_v4[p] = t4[u];
+            a_v4[p] = make_float4(t4[u].x * g.delta_t, t4[u].y * g.delta_t, t4[u].z * g.delta_t,
+                                  t4[u].w * g.delta_t);
             a_base[p] = (v[u] * g.N_r + first[u]) * g.N_t;
             cnt_local += (t4[u].x > 0.f) + (t4[u].y > 0.f) + (t4[u].z > 0.f) + (t4[u].w > 0.f);
           }
@@ -511,8 +533,9 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
         float vals[XT_MAXA];
         xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx, &first,
                           vals);
-        a_v4[p] = make_float4(vals[0], vals[1], vals[2], vals[3]);
-        a_v2[p] = make_float2(vals[4], vals[5]);
+        a_v4[p] = make_float4(vals[0] * g.delta_t, vals[1] * g.delta_t, vals[2] * g.delta_t,
+                              vals[3] * g.delta_t);
+        a_v2[p] = make_float2(vals[4] * g.delta_t, vals[5] * g.delta_t);
         a_base[p] = (v * g.N_r + first) * g.N_t;
 #pragma unroll
         for (int b = 0; b < XT_MAXA; b++) cnt_local += vals[b] > 0.f;
@@ -542,61 +565,56 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     // ---- 2. theta pass
     const int rot = (int)(xt_hash((uint32_t)c * 0x27D4EB2Fu ^ key ^ (uint32_t)L.half) %
                           (uint32_t)(nv > 0 ? nv : 1));
+    const uint32_t lane_off = (uint32_t)zs * sizeof(float);
+    const float T2 = g.T * g.T, Td = g.T * g.delta;
     float th1 = 0.f, th2 = 0.f;
-    for (int q0 = grp; q0 < nv; q0 += U * G) {
-      int base[U];
-      float av[U][NA];
+    if (lane_on) {
+      for (int q0 = grp; q0 < nv; q0 += U * G) {
+        uint32_t off[U];
+        float av[U][NA];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int q = q0 + u * G;
-        base[u] = 0;
+        for (int u = 0; u < U; u++) {
+          const int q = q0 + u * G;
 #pragma unroll
-        for (int b = 0; b < NA; b++) av[u][b] = 0.f;
-        if (q < nv) {
-          int p = q + rot;
-          if (p >= nv) p -= nv;
-          const float4 a4 = a_v4[p];
-          base[u] = a_base[p];
-          av[u][0] = a4.x;
-          av[u][1] = a4.y;
-          av[u][2] = a4.z;
-          av[u][3] = a4.w;
-          if constexpr (!TAB) {
-            const float2 a2 = a_v2[p];
-            av[u][4] = a2.x;
-            av[u][5] = a2.y;
+          for (int b = 0; b < NA; b++) av[u][b] = 0.f;
+          off[u] = OOB;
+          if (q < nv) {
+            int p = q + rot;
+            if (p >= nv) p -= nv;
+            const float4 a4 = a_v4[p];
+            off[u] = (uint32_t)a_base[p] * sizeof(float) + lane_off;
+            av[u][0] = a4.x;
+            av[u][1] = a4.y;
+            av[u][2] = a4.z;
+            av[u][3] = a4.w;
+            if constexpr (!TAB) {
+              const float2 a2 = a_v2[p];
+              av[u][4] = a2.x;
+              av[u][5] = a2.y;
+            }
           }
         }
-      }
-      if (lane_on) {
         float ev[U][NA], wv[U][NA];
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int b = 0; b < NA; b++) {
-            ev[u][b] = 0.f;
-            wv[u][b] = 0.f;
-            if (av[u][b] > 0.f) {
-              const int idx = base[u] + b * g.N_t + zl;
-              ev[u][b] = __hip_atomic_load(&B.e[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              wv[u][b] = B.w[idx];
-            }
+            const uint32_t o = av[u][b] > 0.f ? off[u] : OOB;
+            ev[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_e, o, b * bin_bytes, 0));
+            wv[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_w, o, b * bin_bytes, 0));
           }
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int b = 0; b < NA; b++) {
-            const float pe = av[u][b] * g.delta_t;
-            const float e1 = ev[u][b], w1 = wv[u][b];
-            // selected iff |e*sqrt(w)| < T (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:76)
-            if (e1 * e1 * w1 < g.T * g.T) {
-              th2 += pe * pe * w1;
-              th1 += -(e1 * pe * w1);
-            } else {
-              const float ts = g.T * g.delta * sqrtf(w1) / fabsf(e1);
-              th2 += pe * pe * ts;
-              th1 += -(e1 * pe * ts);
-            }
+            const float pe = av[u][b], e1 = ev[u][b], w1 = wv[u][b];
+            // generalized Huber weight: w if |e*sqrt(w)| < T, else T*delta*sqrt(w)/|e|
+            // (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:38-48,76)
+            const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
+            const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
+            const float pw = pe * ws;
+            th2 += pe * pw;
+            th1 -= e1 * pw;
           }
       }
     }
@@ -604,28 +622,35 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     red2[tid] = th2;
     __syncthreads();
 
-    // ---- theta reduction (wave 0)
-    if (tid < ZL) {
+    // ---- theta reduction by slice
+    if (tid < ZP) {
       float t1 = 0.f, t2 = 0.f;
       for (int q = 0; q < G; q++) {
-        t1 += red1[q * ZL + tid];
-        t2 += red2[q * ZL + tid];
+        t1 += red1[q * ZP + tid];
+        t2 += red2[q * ZP + tid];
       }
-      red1[tid] = t1;
-      red2[tid] = t2;
-      delta[tid] = 0.f;
+      if (zs < ZL) {
+        th1z[zs] = t1;
+        th2z[zs] = t2;
+      }
     }
+    if (tid < ZL) delta[tid] = 0.f;
     if (tid < 64) magb[tid] = 0.f;
     __syncthreads();
 
-    // ---- 3. surrogate update of every slice, red-black in z (or ascending);
-    // the line's own column is xn[4][*] and is updated in place
+    // ---- 3. surrogate update of every active slice, red-black in z (or
+    // ascending); the line's own column is xn[4][*] and is updated in place
     float *xcol = xn + 4 * ZL;
     const int n_steps = L.zorder ? g.N_z : 2;
     for (int step = 0; step < n_steps; step++) {
-      const bool mine = L.zorder ? (zl == step) : ((zl & 1) == step);
-      if (tid < ZL && lane_on && mine) {
-        const int z = zl;
+      const int z = tid;
+      const bool mine = L.zorder ? (z == step) : ((z & 1) == step);
+      bool upd = false;
+      if (z < g.N_z && mine) {
+        upd = par < 0 || (zblock(z) & 1) == par;
+        if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+      }
+      if (upd) {
         const float V = xcol[z];
         float nb[27];
         bool fl[27];
@@ -673,13 +698,13 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
             s1 += bq * tn1;
             s2 += bq;
           }
-          const float T1 = red1[z], T2 = red2[z];
-          const float u = (s1 + (T2 * V) - T1) / (s2 + T2);
+          const float T1 = th1z[z], T2z = th2z[z];
+          const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
           float nv2 = V + g.alpha * (u - V);
           if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
           xcol[z] = nv2;
           delta[z] = nv2 - V;
-          atomicAdd(&magb[zblk], fabsf(nv2 - V));
+          atomicAdd(&magb[zblock(z)], fabsf(nv2 - V));
           vox_mag += fabsf(nv2);
           atomicAdd(&s_perf, 1);
         }
@@ -688,8 +713,8 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     }
 
     // ---- write the line back and its update magnitude
-    if (tid < ZL && zok)
-      __hip_atomic_store(&B.x[((size_t)f * NYX + c) * g.N_z + zl], xcol[zl], __ATOMIC_RELAXED,
+    if (tid < g.N_z)
+      __hip_atomic_store(&B.x[((size_t)f * NYX + c) * g.N_z + tid], xcol[tid], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     if (tid < g.nb) {
       // MagUpdateMap of this line: reset and re-accumulated by the blocks that
@@ -702,42 +727,38 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     }
 
     // ---- 4. e -= A * delta over the same footprint
-    if (zok) {
-      const float dz = delta[zl];
-      if (dz != 0.f) {
-        for (int q = grp; q < nv; q += G) {
-          int p = q + rot;
-          if (p >= nv) p -= nv;
-          const int base = a_base[p];
-          const float4 a4 = a_v4[p];
-          float av[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, 0.f, 0.f};
-          if constexpr (!TAB) {
-            const float2 a2 = a_v2[p];
-            av[4] = a2.x;
-            av[5] = a2.y;
-          }
-#pragma unroll
-          for (int b = 0; b < NA; b++) {
-            const float a = av[b];
-            if (a > 0.f) {
-              const float pe = a * g.delta_t;
-              __hip_atomic_fetch_add(&B.e[base + b * g.N_t + zl], -(pe * dz), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-            }
-          }
+    const float dz = (zs < ZL && lane_on) ? delta[zs] : 0.f;
+    if (dz != 0.f) {
+      for (int q = grp; q < nv; q += G) {
+        int p = q + rot;
+        if (p >= nv) p -= nv;
+        const uint32_t o = (uint32_t)a_base[p] * sizeof(float) + lane_off;
+        const float4 a4 = a_v4[p];
+        float av[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, 0.f, 0.f};
+        if constexpr (!TAB) {
+          const float2 a2 = a_v2[p];
+          av[4] = a2.x;
+          av[5] = a2.y;
         }
+#pragma unroll
+        for (int b = 0; b < NA; b++)
+          __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(-(av[b] * dz), rs_e, av[b] > 0.f ? o : OOB,
+                                                      b * bin_bytes, 0);
       }
     }
     // the next line's loads must see these atomics: wait for all of this
-    // thread's memory operations, then for every thread of the workgroup
+    // thread's memory operations, then for every thread of the workgroup, and
+    // drop any stale copies of e from this CU's cache
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (tid == 0) {
       float sm = 0.f;
       for (int b = 0; b < g.nb; b++) sm += magb[b];
       B.magsum[(size_t)f * NYX + c] = sm;
       n_upd += (double)s_perf;
       n_foot += (double)s_perf * (double)s_cnt_col * (double)g.zo;
+      n_vis += 1.0;
       s_cnt_col = 0;
       s_perf = 0;
     }
@@ -745,10 +766,33 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   }
   // ---- per-workgroup statistics
   if (tid < ZL && L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
-  if (tid == 0 && n_upd != 0.0) {
+  if (tid == 0 && n_vis != 0.0) {
     atomicAdd(&B.stats[1], n_upd);
     atomicAdd(&B.stats[2], n_foot);
+    atomicAdd(&B.stats[3], n_vis);
   }
+}
+
+// Line selection per (frame, z-block parity): the update magnitude summed over
+// the parity's blocks, magpar[f][par][c]; after the selection on it, every block
+// of the parity takes the line's flag: blk[f][b][c] = flag[f][b & 1][c].
+__global__ void k_mag_parity(int nb, size_t NYX, int n_time, const float *__restrict__ mag,
+                             float *__restrict__ magpar) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NYX * n_time) return;
+  const size_t f = i / NYX, c = i % NYX;
+  float a[2] = {0.f, 0.f};
+  for (int b = 0; b < nb; b++) a[b & 1] += mag[(f * nb + b) * NYX + c];
+  magpar[(f * 2 + 0) * NYX + c] = a[0];
+  magpar[(f * 2 + 1) * NYX + c] = a[1];
+}
+
+__global__ void k_flag_blocks(int nb, size_t NYX, int n_time, const uint8_t *__restrict__ flag,
+                              uint8_t *__restrict__ blk) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NYX * n_time) return;
+  const size_t f = i / NYX, c = i % NYX;
+  for (int b = 0; b < nb; b++) blk[(f * nb + b) * NYX + c] = flag[(f * 2 + (b & 1)) * NYX + c];
 }
 
 // NHICD line list of each (frame, parity) map: the selected lines inside the

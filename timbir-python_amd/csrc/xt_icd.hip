@@ -52,19 +52,19 @@ static int set_err(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0};
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 0, 0, 0, 0};  // by XT_OPT_*
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 10) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 12) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 10) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 12) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -177,7 +177,7 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
 // bins 4..5 without the table) and its e offset; theta partials; neighbourhood.
 static size_t icd_lds_bytes(const XtGeom &g, int ZL, bool table) {
   return (size_t)g.max_views * (sizeof(float4) + (table ? 0 : sizeof(float2)) + sizeof(int)) +
-         2 * 256 * sizeof(float) + 12 * ZL * sizeof(float) + 64 * sizeof(float);
+         2 * 256 * sizeof(float) + 14 * ZL * sizeof(float) + 64 * sizeof(float);
 }
 
 template <typename T>
@@ -217,7 +217,7 @@ struct xt_icd_state {
   int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
   uint8_t *line_mask = nullptr;
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
-  float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr;
+  float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr, *magpar = nullptr;
   int *nsel = nullptr, *nsel_out = nullptr;
   double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
   double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
@@ -229,7 +229,8 @@ struct xt_icd_state {
   int64_t graph_launches = 0;   // voxel-update launches in the captured iteration
   int off_nch = 1, off_chunk = 1, off_nblk = 1;
   // statistics
-  double total_updates = 0, total_foot = 0, icd_ms = 0, last_cost = NAN, stop_pct = NAN;
+  double total_updates = 0, total_foot = 0, total_visits = 0, icd_ms = 0, last_cost = NAN,
+         stop_pct = NAN;
   int64_t launches = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev;
@@ -243,6 +244,7 @@ static void free_state(xt_icd_state *s) {
                   s->lines,    s->nhicd_list, s->nhicd_count, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
+                  s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
                   s->at_vals,  s->at_first,   s->iter_dev};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
@@ -284,15 +286,29 @@ static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *
   return XT_OK;
 }
 
+// Packed lanes (the active z-block parity's slices in ZL/2 lanes) need blocks of
+// equal size and an even number of them.
+static bool icd_packable(const XtGeom &g, int ZL, const XtIcdLaunch &L) {
+  return g_opt[XT_OPT_PACK] != 0 && L.mode != 2 && g.nb % 2 == 0 && g.N_z % g.nb == 0;
+}
+
 template <int ZL>
 static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
                        int gx, size_t lds) {
-  if (B.at_vals)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true>), dim3(gx, g.N_time), dim3(256), lds,
-                       st, g, B, L);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false>), dim3(gx, g.N_time), dim3(256), lds,
-                       st, g, B, L);
+  const dim3 grid(gx, g.N_time), block(256);
+  const bool pack = icd_packable(g, ZL, L);
+  if (B.at_vals) {
+    if (pack)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, true>), grid, block, lds, st, g, B, L);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, false>), grid, block, lds, st, g, B, L);
+  } else {
+    if (pack)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true>), grid, block, lds, st, g, B, L);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false>), grid, block, lds, st, g, B,
+                         L);
+  }
 }
 
 static XtIcdBuffers buffers_of(xt_icd_state *s) {
@@ -387,13 +403,30 @@ static int variance_update(xt_icd_state *s) {
 // NHICD selection per (frame, z-block) map as F/XT_NHICD.c:178-205, then the
 // per-line union that decides which lines the NHICD launches visit.
 static int vsc_select(xt_icd_state *s) {
-  const int M = s->g.N_time * s->g.nb;
-  dim3 gf((unsigned)((s->NYX + 255) / 256), M);
-  hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->mag_blk, s->filt);
-  hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
-                     s->nhicd_blk, s->nsel_out);
-  hipLaunchKernelGGL(k_vsc_any, dim3((unsigned)((s->NYX * s->g.N_time + 255) / 256)), dim3(256), 0,
-                     s->st, s->g.nb, s->NYX, s->g.N_time, s->nhicd_blk, s->nhicd_flag);
+  const unsigned nl = (unsigned)((s->NYX * s->g.N_time + 255) / 256);
+  if (g_opt[XT_OPT_NHICD_LINES] != 0) {
+    // one selection per (frame, parity) on the parity's summed map; every
+    // block of the parity takes it
+    const int M = s->g.N_time * 2;
+    hipLaunchKernelGGL(k_mag_parity, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX, s->g.N_time,
+                       s->mag_blk, s->magpar);
+    dim3 gf((unsigned)((s->NYX + 255) / 256), M);
+    hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->magpar,
+                       s->filt);
+    hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
+                       s->nhicd_flag, s->nsel_out);
+    hipLaunchKernelGGL(k_flag_blocks, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX,
+                       s->g.N_time, s->nhicd_flag, s->nhicd_blk);
+  } else {
+    const int M = s->g.N_time * s->g.nb;
+    dim3 gf((unsigned)((s->NYX + 255) / 256), M);
+    hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->mag_blk,
+                       s->filt);
+    hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
+                       s->nhicd_blk, s->nsel_out);
+    hipLaunchKernelGGL(k_vsc_any, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX, s->g.N_time,
+                       s->nhicd_blk, s->nhicd_flag);
+  }
   hipLaunchKernelGGL(k_nhicd_compact, dim3(s->g.N_time * 2), dim3(1024), 0, s->st, (int)s->NYX,
                      s->nhicd_flag, s->line_mask, s->nhicd_list, s->nhicd_count);
   XT_HIP(hipGetLastError());
@@ -575,8 +608,9 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
                         g.max_views));
   s->nsino = (size_t)g.N_p * g.N_r * g.N_t;
   s->NYX = (size_t)g.N_y * g.N_x;
-  if (s->nsino >= ((size_t)1 << 31))
-    return fail(set_err(XT_E_ARG, "sinogram of %zu entries: at most 2^31 - 1", s->nsino));
+  // the voxel update addresses e and w through 32-bit buffer offsets below 2^31
+  if (s->nsino * sizeof(float) >= ((size_t)1 << 31))
+    return fail(set_err(XT_E_ARG, "sinogram of %zu entries: at most 2^29 - 1", s->nsino));
   s->nobj = (size_t)g.N_time * s->NYX * g.N_z;
   // masked lines, F/XT_ICD_update.c:958-974
   std::vector<int> masked;
@@ -596,9 +630,10 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       (rc = dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX)) ||
       (rc = dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX)) ||
-      (rc = dev_alloc(&s->filt, (size_t)g.N_time * g.nb * s->NYX)) ||
-      (rc = dev_alloc(&s->nsel, g.N_time * g.nb)) ||
-      (rc = dev_alloc(&s->nsel_out, g.N_time * g.nb)) ||
+      (rc = dev_alloc(&s->filt, (size_t)g.N_time * (g.nb > 2 ? g.nb : 2) * s->NYX)) ||
+      (rc = dev_alloc(&s->magpar, (size_t)g.N_time * 2 * s->NYX)) ||
+      (rc = dev_alloc(&s->nsel, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
+      (rc = dev_alloc(&s->nsel_out, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
       (rc = dev_alloc(&s->stats, 8)) || (rc = dev_alloc(&s->red_wg, 8192)) ||
       (rc = dev_alloc(&s->scal, 8)) || (rc = dev_alloc(&s->iter_dev, 1)))
     return fail(rc);
@@ -633,7 +668,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   {
     // NHICD lines per (frame, z-block): floor(UpdateSelectNum / NHICD_Iterations),
     // F/XT_NHICD.c:182 (every block holds all masked lines)
-    std::vector<int> ns((size_t)g.N_time * g.nb, s->n_masked / s->K);
+    std::vector<int> ns((size_t)g.N_time * (g.nb > 2 ? g.nb : 2), s->n_masked / s->K);
     if (hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(XT_E_HIP, "nsel upload"));
   }
@@ -758,19 +793,21 @@ static int iterate_one(xt_icd_state *s) {
                      (size_t)s->g.N_time * s->NYX, 0, s->red_wg);
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 4);
   XT_HIP(hipGetLastError());
-  double st[3], avg = 0;
-  XT_HIP(hipMemcpyAsync(st, s->stats, 3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  double st[4], avg = 0;
+  XT_HIP(hipMemcpyAsync(st, s->stats, 4 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   XT_HIP(hipMemcpyAsync(&avg, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->st));
   XT_HIP(hipStreamSynchronize(s->st));
   XT_TRY(collect_timing(s));
   s->total_updates += st[1];
   s->total_foot += st[2];
+  s->total_visits += st[3];
   s->stop_pct = 100.0 * avg / st[0];
   s->stopped = (s->stop_pct < (double)s->stop_threshold) ? 1 : 0;
   int every = (int)g_opt[XT_OPT_RESYNC_EVERY];
   if (every > 0 && Iter % every == 0) XT_TRY(icd_resync(s));
   if (g_opt[XT_OPT_COST] != 0) XT_TRY(xt_icd_cost(s, &s->last_cost));
-  dbg(s->T, "ICD iteration %d: updates %.0f, average update %.6f%%, cost %.9g\n", Iter, st[1],
+  dbg(s->T, "ICD iteration %d: updates %.0f in %.0f line visits, average update %.6f%%, cost %.9g\n",
+      Iter, st[1], st[3],
       s->stop_pct, s->last_cost);
   return XT_OK;
 }
@@ -792,6 +829,7 @@ extern "C" int xt_icd_get_stats(xt_icd_state *s, xt_icd_stats *out) {
   out->last_cost = s->last_cost;
   out->stop_percentage = s->stop_pct;
   out->stopped = s->stopped;
+  out->line_visits = s->total_visits;
   return XT_OK;
 }
 
