@@ -39,7 +39,18 @@ struct XtGeom {
   float tfilt;     // Time_Filter[0]
   float alpha, T, delta;  // over-relaxation, generalized-Huber T and delta
   int positivity;
+  // sinogram layout in HBM (e, w, y): pm = 0: [N_p][N_r][N_t]; pm = 1: split by
+  // z-block parity, [2][N_p][N_r][H] with H = N_t/2, detector row t in plane
+  // ((t / bs) & 1) at position (t / (2 bs)) * bs + t % bs (bs slices per block)
+  int pm, bs, H;
 };
+
+// Offset of sinogram entry (view p, bin r, row t) in the HBM layout.
+__device__ __forceinline__ size_t xt_sidx(const XtGeom &g, int p, int r, int t) {
+  if (!g.pm) return ((size_t)p * g.N_r + r) * g.N_t + t;
+  const int par = (t / g.bs) & 1, tt = (t / (2 * g.bs)) * g.bs + t % g.bs;
+  return (((size_t)par * g.N_p + p) * g.N_r + r) * g.H + tt;
+}
 
 // ---------------------------------------------------------------------------
 // System-matrix column for (row, col, view): the exact float steps of
@@ -241,10 +252,10 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
   }
   if (!lane_ok) return;
   // z_overlap_num == 1: slice zl is detector row t = zl
-  const size_t idx = ((size_t)v * g.N_r + j) * g.N_t + zl;
   if (mode == 0) {
-    out64[idx] = acc;
+    out64[((size_t)v * g.N_r + j) * g.N_t + zl] = acc;
   } else {
+    const size_t idx = xt_sidx(g, v, j, zl);
     double ev = (double)y[idx] - acc - (double)d[(size_t)j * g.N_t + zl];
     e[idx] = (float)ev;
   }
@@ -474,7 +485,10 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   const __amdgpu_buffer_rsrc_t rs_e = __builtin_amdgcn_make_buffer_rsrc(B.e, 0, sino_bytes, 0x00020000);
   const __amdgpu_buffer_rsrc_t rs_w =
       __builtin_amdgcn_make_buffer_rsrc((void *)B.w, 0, sino_bytes, 0x00020000);
-  const int bin_bytes = g.N_t * (int)sizeof(float);
+  const int bin_bytes = (g.pm ? g.H : g.N_t) * (int)sizeof(float);
+  // first entry of the launch's parity plane (split layout)
+  const int plane = g.pm ? par * g.N_p * g.N_r * g.H : 0;
+  const int row_len = g.pm ? g.H : g.N_t;
 
   // homogeneous lines: positions blockIdx.x, +gridDim.x, ... of the random order;
   // NHICD lines: a contiguous run of the raster-ordered list per workgroup (the
@@ -521,7 +535,7 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
           if (p < nv) {
             a_v4[p] = make_float4(t4[u].x * g.delta_t, t4[u].y * g.delta_t, t4[u].z * g.delta_t,
                                   t4[u].w * g.delta_t);
-            a_base[p] = (v[u] * g.N_r + first[u]) * g.N_t;
+            a_base[p] = plane + (v[u] * g.N_r + first[u]) * row_len;
             cnt_local += (t4[u].x > 0.f) + (t4[u].y > 0.f) + (t4[u].z > 0.f) + (t4[u].w > 0.f);
           }
         }
@@ -536,7 +550,7 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
         a_v4[p] = make_float4(vals[0] * g.delta_t, vals[1] * g.delta_t, vals[2] * g.delta_t,
                               vals[3] * g.delta_t);
         a_v2[p] = make_float2(vals[4] * g.delta_t, vals[5] * g.delta_t);
-        a_base[p] = (v * g.N_r + first) * g.N_t;
+        a_base[p] = plane + (v * g.N_r + first) * row_len;
 #pragma unroll
         for (int b = 0; b < XT_MAXA; b++) cnt_local += vals[b] > 0.f;
       }
@@ -565,7 +579,8 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     // ---- 2. theta pass
     const int rot = (int)(xt_hash((uint32_t)c * 0x27D4EB2Fu ^ key ^ (uint32_t)L.half) %
                           (uint32_t)(nv > 0 ? nv : 1));
-    const uint32_t lane_off = (uint32_t)zs * sizeof(float);
+    // lane's entry in a row: its slice, or in the split layout its packed position
+    const uint32_t lane_off = (uint32_t)(g.pm ? lz : zs) * sizeof(float);
     const float T2 = g.T * g.T, Td = g.T * g.delta;
     float th1 = 0.f, th2 = 0.f;
     if (lane_on) {
@@ -843,8 +858,9 @@ __global__ void k_offset_partial(XtGeom g, const float *__restrict__ e, const fl
   if (rt >= NRT) return;
   const int p0 = ch * chunk_views, p1 = min(g.N_p, p0 + chunk_views);
   double num = 0.0, den = 0.0;
+  const int r = rt / g.N_t, t = rt - (rt / g.N_t) * g.N_t;
   for (int p = p0; p < p1; p++) {
-    const size_t idx = (size_t)p * NRT + rt;
+    const size_t idx = xt_sidx(g, p, r, t);
     const float ev = e[idx], wv = w[idx];
     if (ev * ev * wv < g.T * g.T) {
       num += (double)ev * (double)wv;
@@ -925,10 +941,8 @@ __global__ void k_offset_apply(XtGeom g, int chunk_views, const double *__restri
   const float step = (float)((numden[rt] - *gamma) / numden[NRT + rt]);
   if (ch == 0) d[rt] = d[rt] + step;
   const int p0 = ch * chunk_views, p1 = min(g.N_p, p0 + chunk_views);
-  for (int p = p0; p < p1; p++) {
-    const size_t idx = (size_t)p * NRT + rt;
-    e[idx] -= step;
-  }
+  const int r = rt / g.N_t, t = rt - (rt / g.N_t) * g.N_t;
+  for (int p = p0; p < p1; p++) e[xt_sidx(g, p, r, t)] -= step;
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,11 +1198,29 @@ __global__ void k_object_to_host(XtGeom g, const float *__restrict__ x,
 }
 
 // ProjSelect from the final e and w (F/XT_ICD_update.c:577).
+// ProjSelect in the host's [N_p][N_r][N_t] order.
 __global__ void k_select(XtGeom g, const float *__restrict__ e, const float *__restrict__ w, size_t n,
                          uint8_t *__restrict__ sel) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    sel[i] = (fabs((double)e[i] * sqrt((double)w[i])) < (double)g.T) ? 1 : 0;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % g.N_t), r = (int)((i / g.N_t) % g.N_r), p = (int)(i / ((size_t)g.N_t * g.N_r));
+    const size_t k = xt_sidx(g, p, r, t);
+    sel[i] = (fabs((double)e[k] * sqrt((double)w[k])) < (double)g.T) ? 1 : 0;
+  }
+}
+
+// Between the host's [N_p][N_r][N_t] order and the HBM layout.
+__global__ void k_sino_layout(XtGeom g, const float *__restrict__ src, float *__restrict__ dst,
+                              size_t n, int to_hbm) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % g.N_t), r = (int)((i / g.N_t) % g.N_r), p = (int)(i / ((size_t)g.N_t * g.N_r));
+    const size_t k = xt_sidx(g, p, r, t);
+    if (to_hbm)
+      dst[k] = src[i];
+    else
+      dst[i] = src[k];
+  }
 }
 
 __global__ void k_zero_f64(double *p, int n) {

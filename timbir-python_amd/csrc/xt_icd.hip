@@ -144,6 +144,9 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
   g->zo = S->z_overlap_num;
   g->nb = (T && T->num_z_blocks > 0) ? T->num_z_blocks : 1;
   if (g->nb > 64) g->nb = 64;
+  g->pm = 0;  // [N_p][N_r][N_t] unless the ICD state splits it by parity
+  g->bs = O->N_z / g->nb > 0 ? O->N_z / g->nb : 1;
+  g->H = S->N_t;
   g->delta_r = S->delta_r;
   g->delta_t = S->delta_t;
   g->R0 = S->R0;
@@ -289,7 +292,8 @@ static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *
 // Packed lanes (the active z-block parity's slices in ZL/2 lanes) need blocks of
 // equal size and an even number of them.
 static bool icd_packable(const XtGeom &g, int ZL, const XtIcdLaunch &L) {
-  return g_opt[XT_OPT_PACK] != 0 && L.mode != 2 && g.nb % 2 == 0 && g.N_z % g.nb == 0;
+  (void)ZL;
+  return g.pm != 0 && L.mode != 2;
 }
 
 template <int ZL>
@@ -496,6 +500,21 @@ static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject 
   return XT_OK;
 }
 
+// A host [N_p][N_r][N_t] array into dst in the HBM layout (through e when the
+// layout is split by parity; e is set afterwards by the resync).
+static int upload_sino(xt_icd_state *s, float *dst, const float *host) {
+  if (!s->g.pm) {
+    XT_HIP(hipMemcpy(dst, host, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+    return XT_OK;
+  }
+  XT_HIP(hipMemcpy(s->e, host, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sino_layout, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->e, dst,
+                     s->nsino, 1);
+  XT_HIP(hipGetLastError());
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
 static int upload_object(xt_icd_state *s, const ScannedObject *O, float *x_dev) {
   const XtGeom &g = s->g;
   size_t per = (size_t)g.N_z * g.N_y * g.N_x;
@@ -621,6 +640,12 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       if (xc * xc + yc * yc < T->radius_obj * T->radius_obj) masked.push_back(j * g.N_x + k);
     }
   s->n_masked = (int)masked.size();
+  // split the sinogram by z-block parity when the passes can pack their lanes
+  if (g_opt[XT_OPT_PACK] != 0 && g.nb % 2 == 0 && g.N_z % g.nb == 0 && g.N_t % 2 == 0) {
+    g.pm = 1;
+    g.bs = g.N_z / g.nb;
+    g.H = g.N_t / 2;
+  }
   if ((rc = setup_lines(s, masked))) return fail(rc);
   if ((rc = upload_views(s, S, O))) return fail(rc);
   if ((rc = dev_alloc(&s->e, s->nsino)) || (rc = dev_alloc(&s->w, s->nsino)) ||
@@ -653,11 +678,11 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
         (rc = dev_alloc(&s->off_wg, (size_t)2 * s->off_nblk)))
       return fail(rc);
   }
-  if (hipMemcpy(s->y, &S->Projection[0][0][0], s->nsino * sizeof(float), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      hipMemcpy(s->w, &T->Weight[0][0][0], s->nsino * sizeof(float), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t * sizeof(float),
+  // the host's y and w, put into the HBM layout through e (set later by the resync)
+  if ((rc = upload_sino(s, s->y, &S->Projection[0][0][0])) ||
+      (rc = upload_sino(s, s->w, &T->Weight[0][0][0])))
+    return fail(rc);
+  if (hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t * sizeof(float),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(XT_E_HIP, "sinogram upload failed"));
   if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
@@ -853,7 +878,19 @@ extern "C" int xt_icd_download(xt_icd_state *s, Sinogram *S, ScannedObject *O, T
   XT_HIP(hipMemcpy(&S->ProjOffset[0][0], s->d, (size_t)s->g.N_r * s->g.N_t * sizeof(float),
                    hipMemcpyDeviceToHost));
   if (s->update_var == 1) {
-    XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+    if (s->g.pm) {
+      float *tmp = nullptr;
+      XT_TRY(dev_alloc(&tmp, s->nsino));
+      hipLaunchKernelGGL(k_sino_layout, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->w,
+                         tmp, s->nsino, 0);
+      hipError_t e = hipStreamSynchronize(s->st);
+      if (e == hipSuccess)
+        e = hipMemcpy(&T->Weight[0][0][0], tmp, s->nsino * sizeof(float), hipMemcpyDeviceToHost);
+      (void)hipFree(tmp);
+      if (e != hipSuccess) return set_err(XT_E_HIP, "Weight download: %s", hipGetErrorString(e));
+    } else {
+      XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+    }
     T->var_est = s->var_est;
   }
   uint8_t *sel = nullptr;
