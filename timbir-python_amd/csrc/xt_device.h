@@ -23,6 +23,7 @@
 #define XT_MAXA 6          // max detector bins of one A column (2*delta_xy/delta_r + 2)
 #define XT_NBINS 64        // DETECTOR_RESPONSE_BINS
 #define XT_HR_STRIDE 65
+#define XT_FP_BINS 64      // detector bins per forward-projection workgroup
 #define XT_PI 3.14159265358979323846
 #define XT_PI_2 1.57079632679489661923132169163975144
 #define XT_PI_4 0.785398163397448309615660845819875721
@@ -96,32 +97,6 @@ __device__ __forceinline__ int xt_amatrix_column(const XtGeom &g, const float *_
   return count;
 }
 
-// A entry of bin j for voxel (row, col) and one view, or 0 when bin j is not
-// in that column (same steps as xt_amatrix_column, for the ray-driven gather).
-__device__ __forceinline__ float xt_amatrix_entry(const XtGeom &g, const float *__restrict__ H,
-                                                  float cosv, float sinv, int row, int col, int j) {
-#pragma clang fp contract(off)
-  float x = (float)((double)g.x0 + ((double)(float)col + 0.5) * (double)g.delta_xy);
-  float y = (float)((double)g.y0 + ((double)(float)row + 0.5) * (double)g.delta_xy);
-  float r = x * cosv - y * sinv;
-  float rmin = r - g.delta_xy;
-  float rmax = r + g.delta_xy;
-  if (rmax < g.R0 || rmin > g.RMax) return 0.f;
-  int imin = (int)floorf((rmin - g.R0) / g.delta_r);
-  int imax = (int)floorf((rmax - g.R0) / g.delta_r);
-  if (imax >= g.N_r) imax = g.N_r - 1;
-  if (imin < 0) imin = 0;
-  if (j < imin || j > imax) return 0.f;
-  float Rc = (float)((double)g.R0 + ((double)(float)j + 0.5) * (double)g.delta_r);
-  float dr = fabsf(r - Rc);
-  int k = (int)floorf(dr / g.OffsetR);
-  if (k < 0 || k >= XT_NBINS) return 0.f;
-  float w1 = dr - (float)k * g.OffsetR;
-  float w2 = g.OffsetR - w1;
-  float f1 = (w2 * H[k] + w1 * H[k + 1]) / g.OffsetR;
-  return f1 > 0 ? f1 : 0.f;
-}
-
 // ---------------------------------------------------------------------------
 // Detector response table, one workgroup per view: the trapezoidal voxel
 // profile (F/XT_Profile.c:46-103) convolved with the unit-area Hamming beam
@@ -188,10 +163,12 @@ __global__ void __launch_bounds__(256) k_detector_response(XtGeom g, const float
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection by rays: one thread per (view, bin, slice), FP64 sum over
-// every voxel whose A column for that view holds the bin.  Candidates are
-// enumerated along the axis more perpendicular to the ray (at most ~6 per
-// row or column); each is accepted by the exact column test.
+// Forward projection (F/XT_ForwardProject.c:8-25 summed over all voxels):
+// one workgroup per (view, chunk of XT_FP_BINS bins) with FP64 accumulators
+// [bin][slice] in LDS.  Its view groups walk the rows (or columns, whichever
+// axis is more perpendicular to the ray) and enumerate the voxels whose
+// column can reach the chunk; each voxel's exact A column is built once, and
+// its entries times the line's slices are added to the accumulators.
 // mode 0: out64 = Ax.   mode 1: e = y - Ax - d (float).
 template <int ZL>
 __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *__restrict__ x,
@@ -203,61 +180,101 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
                                                          const float *__restrict__ y,
                                                          const float *__restrict__ d,
                                                          float *__restrict__ e) {
-  constexpr int RB = 256 / ZL;
-  const int v = blockIdx.y;
-  const int zl = threadIdx.x % ZL, jb = threadIdx.x / ZL;
-  const int j = blockIdx.x * RB + jb;
-  const bool lane_ok = (j < g.N_r) && (zl < g.N_z);
+  constexpr int G = 256 / ZL;
+  __shared__ double acc[XT_FP_BINS * ZL];
+  __shared__ float cval[G][ZL][XT_MAXA];
+  __shared__ int cfirst[G][ZL];
+  const int v = blockIdx.y, j0 = blockIdx.x * XT_FP_BINS;
+  const int tid = threadIdx.x, zl = tid % ZL, grp = tid / ZL;
+  for (int q = tid; q < XT_FP_BINS * ZL; q += 256) acc[q] = 0.0;
+  __syncthreads();
   const int f = view_frame[v];
-  double acc = 0.0;
-  if (j < g.N_r && f >= 0) {
+  if (f >= 0) {
     const float c = cosv[v], s = sinv[v];
     const float *H = Hr + (size_t)v * XT_HR_STRIDE;
     const float *xf = x + (size_t)f * g.N_y * g.N_x * g.N_z;
-    // accepted voxels satisfy |r - Rc| <= delta_xy + delta_r/2; widen slightly
-    const double Rc = (double)g.R0 + ((double)j + 0.5) * (double)g.delta_r;
+    // voxels whose column can reach bins j0..j0+XT_FP_BINS-1: projected centre within
+    // delta_xy (+ half a bin, + a margin for float rounding) of the chunk's bin centres
+    const int j1 = min(g.N_r, j0 + XT_FP_BINS) - 1;
     const double W = (double)g.delta_xy + 0.5 * (double)g.delta_r + 1e-3 * (double)g.delta_r;
+    const double lo = (double)g.R0 + ((double)j0 + 0.5) * (double)g.delta_r - W;
+    const double hi = (double)g.R0 + ((double)j1 + 0.5) * (double)g.delta_r + W;
     const double cd = c, sd = s, dxy = g.delta_xy;
-    if (fabs(cd) >= fabs(sd)) {
-      for (int row = 0; row < g.N_y; row++) {
-        double yc = (double)g.y0 + (row + 0.5) * dxy;
-        double xa = (Rc - W + yc * sd) / cd, xb = (Rc + W + yc * sd) / cd;
-        if (xa > xb) { double tmp = xa; xa = xb; xb = tmp; }
-        int c0 = (int)floor((xa - (double)g.x0) / dxy - 0.5);
-        int c1 = (int)ceil((xb - (double)g.x0) / dxy - 0.5);
-        if (c0 < 0) c0 = 0;
-        if (c1 > g.N_x - 1) c1 = g.N_x - 1;
-        for (int col = c0; col <= c1; col++) {
-          float a = xt_amatrix_entry(g, H, c, s, row, col, j);
-          if (a != 0.f && lane_ok)
-            acc += (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl] * (double)a;
-        }
+    const bool by_row = fabs(cd) >= fabs(sd);
+    const int n_outer = by_row ? g.N_y : g.N_x;
+    // each view group walks its own rows (or columns); its lanes build the
+    // columns of up to ZL candidate voxels at once, then add their products with
+    // the line's slices into the block's FP64 accumulators
+    for (int o = grp; o < n_outer; o += G) {
+      int k0, k1;
+      if (by_row) {
+        const double yc = (double)g.y0 + (o + 0.5) * dxy;
+        double xa = (lo + yc * sd) / cd, xb = (hi + yc * sd) / cd;
+        if (xa > xb) { const double t = xa; xa = xb; xb = t; }
+        k0 = (int)floor((xa - (double)g.x0) / dxy - 0.5);
+        k1 = (int)ceil((xb - (double)g.x0) / dxy - 0.5);
+        if (k0 < 0) k0 = 0;
+        if (k1 > g.N_x - 1) k1 = g.N_x - 1;
+      } else {
+        const double xc = (double)g.x0 + (o + 0.5) * dxy;
+        double ya = (xc * cd - hi) / sd, yb = (xc * cd - lo) / sd;
+        if (ya > yb) { const double t = ya; ya = yb; yb = t; }
+        k0 = (int)floor((ya - (double)g.y0) / dxy - 0.5);
+        k1 = (int)ceil((yb - (double)g.y0) / dxy - 0.5);
+        if (k0 < 0) k0 = 0;
+        if (k1 > g.N_y - 1) k1 = g.N_y - 1;
       }
-    } else {
-      for (int col = 0; col < g.N_x; col++) {
-        double xc = (double)g.x0 + (col + 0.5) * dxy;
-        double ya = (xc * cd - (Rc + W)) / sd, yb = (xc * cd - (Rc - W)) / sd;
-        if (ya > yb) { double tmp = ya; ya = yb; yb = tmp; }
-        int r0 = (int)floor((ya - (double)g.y0) / dxy - 0.5);
-        int r1 = (int)ceil((yb - (double)g.y0) / dxy - 0.5);
-        if (r0 < 0) r0 = 0;
-        if (r1 > g.N_y - 1) r1 = g.N_y - 1;
-        for (int row = r0; row <= r1; row++) {
-          float a = xt_amatrix_entry(g, H, c, s, row, col, j);
-          if (a != 0.f && lane_ok)
-            acc += (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl] * (double)a;
+      for (int kb = k0; kb <= k1; kb += ZL) {
+        {
+          const int k = kb + zl;
+          int first = 0;
+          float vals[XT_MAXA];
+#pragma unroll
+          for (int b = 0; b < XT_MAXA; b++) vals[b] = 0.f;
+          if (k <= k1) {
+            const int row = by_row ? o : k, col = by_row ? k : o;
+            xt_amatrix_column(g, H, c, s, row, col, &first, vals);
+          }
+          cfirst[grp][zl] = first;
+#pragma unroll
+          for (int b = 0; b < XT_MAXA; b++) cval[grp][zl][b] = vals[b];
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int nk = min(ZL, k1 - kb + 1);
+        if (zl < g.N_z) {
+          for (int m = 0; m < nk; m++) {
+            const int k = kb + m;
+            const int row = by_row ? o : k, col = by_row ? k : o;
+            const double xv = (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl];
+            const int first = cfirst[grp][m];
+#pragma unroll
+            for (int b = 0; b < XT_MAXA; b++) {
+              const float a = cval[grp][m][b];
+              const int jj = first + b - j0;
+              if (a != 0.f && jj >= 0 && jj < XT_FP_BINS) atomicAdd(&acc[jj * ZL + zl], xv * (double)a);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       }
     }
   }
-  if (!lane_ok) return;
-  // z_overlap_num == 1: slice zl is detector row t = zl
-  if (mode == 0) {
-    out64[((size_t)v * g.N_r + j) * g.N_t + zl] = acc;
-  } else {
-    const size_t idx = xt_sidx(g, v, j, zl);
-    double ev = (double)y[idx] - acc - (double)d[(size_t)j * g.N_t + zl];
-    e[idx] = (float)ev;
+  __syncthreads();
+  // z_overlap_num == 1: slice z is detector row t = z
+  for (int q = tid; q < XT_FP_BINS * ZL; q += 256) {
+    const int jj = q / ZL, z = q - jj * ZL, j = j0 + jj;
+    if (j >= g.N_r || z >= g.N_z) continue;
+    const double a = acc[q];
+    if (mode == 0) {
+      out64[((size_t)v * g.N_r + j) * g.N_t + z] = a;
+    } else {
+      const size_t idx = xt_sidx(g, v, j, z);
+      e[idx] = (float)((double)y[idx] - a - (double)d[(size_t)j * g.N_t + z]);
+    }
   }
 }
 

@@ -270,8 +270,7 @@ template <int ZL>
 static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const float *Hr,
                       const float *c, const float *s, const int *vf, int mode, double *out64,
                       const float *y, const float *d, float *e) {
-  constexpr int RB = 256 / ZL;
-  dim3 grid((g.N_r + RB - 1) / RB, g.N_p);
+  dim3 grid((g.N_r + XT_FP_BINS - 1) / XT_FP_BINS, g.N_p);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL>), grid, dim3(256), 0, st, g, x, Hr, c, s,
                      vf, mode, out64, y, d, e);
 }
