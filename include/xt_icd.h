@@ -87,7 +87,8 @@ enum {
   XT_OPT_NHICD_LINES = 12    /* 0: NHICD selects per z-block as the reference does
                                 (F/XT_NHICD.c:178-205); 1: per z-block parity on the
                                 parity's summed update map, so a selected line is updated in
-                                every block of the parity (default 0) */
+                                every block of the parity: the same number of voxel updates
+                                in fewer, fuller line visits (default 1) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);

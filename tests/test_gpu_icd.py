@@ -53,11 +53,16 @@ def run_both(cfg, scan_root, oracle32):
     return p_gpu, p_cpu, info
 
 
-@pytest.mark.parametrize("cfg", [dataclasses.replace(SMALL, iterations=40),
-                                 dataclasses.replace(SMALL4D, iterations=30)],
-                         ids=["small3d", "small4d"])
-def test_converged_parity(cfg, scan_root, oracle32, oracle64a):
-    p_gpu, p_cpu, info = run_both(cfg, scan_root, oracle32)
+@pytest.mark.parametrize("cfg,nhicd_lines", [(dataclasses.replace(SMALL, iterations=40), 1),
+                                              (dataclasses.replace(SMALL, iterations=40), 0),
+                                              (dataclasses.replace(SMALL4D, iterations=30), 1)],
+                         ids=["small3d", "small3d-nhicd-per-block", "small4d"])
+def test_converged_parity(cfg, nhicd_lines, scan_root, oracle32, oracle64a):
+    xt_hip.set_option(xt_hip.OPT_NHICD_LINES, nhicd_lines)
+    try:
+        p_gpu, p_cpu, info = run_both(cfg, scan_root, oracle32)
+    finally:
+        xt_hip.set_option(xt_hip.OPT_NHICD_LINES, 1)
     xg, xc = read_object(p_gpu, cfg), read_object(p_cpu, cfg)
     ph = info["phantom"]
     mean_att = ph[ph > 0].mean()

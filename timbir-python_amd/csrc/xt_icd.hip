@@ -52,7 +52,7 @@ static int set_err(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 0, 0, 0, 0};  // by XT_OPT_*
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 0, 0};  // by XT_OPT_*
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
