@@ -84,11 +84,14 @@ enum {
   XT_OPT_PACK = 11,          /* 1: a pass over one z-block parity packs that parity's slices
                                 into half the lanes when the blocks are of equal size and
                                 even in number; 0: one lane per slice (default 1) */
-  XT_OPT_NHICD_LINES = 12    /* 0: NHICD selects per z-block as the reference does
+  XT_OPT_NHICD_LINES = 12,   /* 0: NHICD selects per z-block as the reference does
                                 (F/XT_NHICD.c:178-205); 1: per z-block parity on the
                                 parity's summed update map, so a selected line is updated in
                                 every block of the parity: the same number of voxel updates
                                 in fewer, fuller line visits (default 1) */
+  XT_OPT_INIT_FBP = 13       /* 1: with initICD 0, ICD_BackProject starts from the filtered
+                                backprojection of y - d instead of zero (no counterpart in
+                                the reference; default 0) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);
@@ -123,6 +126,15 @@ int xt_icd_set_timing(xt_icd_state *state, int32_t on);
 int xt_icd_get_stats(xt_icd_state *state, xt_icd_stats *stats);
 int xt_icd_cost(xt_icd_state *state, double *cost);          /* FP64 MAP cost now */
 int xt_icd_resync(xt_icd_state *state);                       /* FP64 e = y - Ax - d now */
+/* Replace the object by the filtered backprojection of y - d (per frame: ramp
+ * filter of sample spacing delta_r after a 3-point median along the bins that
+ * removes isolated zingers; backprojection with the A-matrix geometry and the
+ * views' angular weights), then recompute e.  When the ring offsets are
+ * estimated (updateProjOffset > 0), d is first set to a guess: the view-mean of
+ * y without outliers minus its 15-bin running mean along the bins (zero-mean if
+ * enforced).
+ * A warm start for ICD with no counterpart in the reference. */
+int xt_icd_fbp(xt_icd_state *state);
 int xt_icd_download(xt_icd_state *state, Sinogram *S, ScannedObject *O, TomoInputs *T);
 int xt_icd_synchronize(xt_icd_state *state);
 void xt_icd_destroy(xt_icd_state *state);

@@ -53,8 +53,8 @@ def run_both(cfg, scan_root, oracle32):
     return p_gpu, p_cpu, info
 
 
-@pytest.mark.parametrize("cfg,nhicd_lines", [(dataclasses.replace(SMALL, iterations=40), 1),
-                                              (dataclasses.replace(SMALL, iterations=40), 0),
+@pytest.mark.parametrize("cfg,nhicd_lines", [(dataclasses.replace(SMALL, iterations=150), 1),
+                                              (dataclasses.replace(SMALL, iterations=150), 0),
                                               (dataclasses.replace(SMALL4D, iterations=30), 1)],
                          ids=["small3d", "small3d-nhicd-per-block", "small4d"])
 def test_converged_parity(cfg, nhicd_lines, scan_root, oracle32, oracle64a):

@@ -279,6 +279,149 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
 }
 
 // ---------------------------------------------------------------------------
+// Filtered backprojection (a warm start for ICD; the reference has none).
+// Rows p = y - d of each (view, detector row) are cleaned of isolated outliers
+// (zingers) by a 3-point median along the bins and filtered with the Ram-Lak
+// ramp of sample spacing delta_r: q_j = (1/delta_r) (p_j/4 - sum_{n odd}
+// (p_{j-n} + p_{j+n}) / (n pi)^2); q is written [N_p][N_r][N_t].
+__global__ void __launch_bounds__(256) k_fbp_filter(XtGeom g, const float *__restrict__ y,
+                                                    const float *__restrict__ d,
+                                                    float *__restrict__ q) {
+  constexpr int TB = 4;  // detector rows per workgroup
+  extern __shared__ float fbuf[];
+  float *raw = fbuf, *med = fbuf + TB * g.N_r, *coef = med + TB * g.N_r;  // coef[(n-1)/2]
+  const int v = blockIdx.y, t0 = blockIdx.x * TB, tid = threadIdx.x, NR = g.N_r;
+  for (int i = tid; i < (NR + 1) / 2; i += blockDim.x) {
+    const double n = 2.0 * i + 1.0;
+    coef[i] = (float)(1.0 / (n * n * XT_PI * XT_PI));
+  }
+  for (int i = tid; i < TB * NR; i += blockDim.x) {
+    const int k = i / NR, r = i - k * NR, t = t0 + k;
+    raw[i] = (t < g.N_t) ? y[xt_sidx(g, v, r, t)] - d[(size_t)r * g.N_t + t] : 0.f;
+  }
+  __syncthreads();
+  for (int i = tid; i < TB * NR; i += blockDim.x) {
+    const int r = i % NR;
+    const float b = raw[i];
+    if (r == 0 || r == NR - 1) {
+      med[i] = b;
+    } else {
+      const float a = raw[i - 1], c = raw[i + 1];
+      med[i] = fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < TB * NR; i += blockDim.x) {
+    const int k = i % TB, j = i / TB, t = t0 + k;
+    if (t >= g.N_t) continue;
+    const float *m = med + k * NR;
+    float acc = 0.25f * m[j];
+    for (int n = 1; n < NR; n += 2) {
+      float pr = 0.f;
+      if (j - n >= 0) pr += m[j - n];
+      if (j + n < NR) pr += m[j + n];
+      acc -= pr * coef[n >> 1];
+    }
+    q[((size_t)v * NR + j) * g.N_t + t] = acc / g.delta_r;
+  }
+}
+
+// Ring-offset guess for the FBP start: per detector pixel the mean over views
+// of y without outliers (values beyond 4 standard deviations of the plain mean:
+// zingers), minus its 15-bin running mean along the bins (the view-averaged
+// projection of an object is smooth in r, per-bin offsets are not).
+__global__ void k_offset_guess_mean(XtGeom g, const float *__restrict__ y, float *__restrict__ m) {
+  const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rt >= g.N_r * g.N_t) return;
+  const int r = rt / g.N_t, t = rt - r * g.N_t;
+  double s1 = 0.0, s2 = 0.0;
+  for (int p = 0; p < g.N_p; p++) {
+    const double v = y[xt_sidx(g, p, r, t)];
+    s1 += v;
+    s2 += v * v;
+  }
+  const double mu = s1 / g.N_p, sd = sqrt(fmax(s2 / g.N_p - mu * mu, 0.0));
+  double acc = 0.0;
+  int n = 0;
+  for (int p = 0; p < g.N_p; p++) {
+    const double v = y[xt_sidx(g, p, r, t)];
+    if (fabs(v - mu) <= 4.0 * sd) {
+      acc += v;
+      n++;
+    }
+  }
+  m[rt] = (float)(n ? acc / n : mu);
+}
+
+__global__ void __launch_bounds__(1024) k_offset_guess_apply(XtGeom g, const float *__restrict__ m,
+                                                             int zero_mean, float *__restrict__ d) {
+  __shared__ double part[1024];
+  const int n = g.N_r * g.N_t, tid = threadIdx.x;
+  double acc = 0.0;
+  for (int rt = tid; rt < n; rt += blockDim.x) {
+    const int r = rt / g.N_t, t = rt - r * g.N_t;
+    double trend = 0.0;
+    for (int q = r - 7; q <= r + 7; q++) {
+      const int qc = q < 0 ? 0 : (q >= g.N_r ? g.N_r - 1 : q);
+      trend += m[qc * g.N_t + t];
+    }
+    const float v = (float)(m[rt] - trend / 15.0);
+    d[rt] = v;
+    acc += v;
+  }
+  part[tid] = acc;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (tid < h) part[tid] += part[tid + h];
+    __syncthreads();
+  }
+  if (zero_mean) {
+    const float mean = (float)(part[0] / n);
+    for (int rt = tid; rt < n; rt += blockDim.x) d[rt] -= mean;
+  }
+}
+
+// x(line, z) = sum over the frame's views of wv[v] * q_v(u), with u = (r - R0) /
+// delta_r - 1/2 the bin coordinate of the voxel centre's projection r (the
+// A-matrix geometry, F/XT_AMatrix.c:59-62) and linear interpolation; wv are the
+// views' angular weights (sum pi over a frame).
+template <int ZL>
+__global__ void __launch_bounds__(256) k_fbp_backproject(XtGeom g, const float *__restrict__ q,
+                                                         const float *__restrict__ wv,
+                                                         const float *__restrict__ cosv,
+                                                         const float *__restrict__ sinv,
+                                                         const int *__restrict__ views,
+                                                         const int *__restrict__ proj_num,
+                                                         const uint8_t *__restrict__ mask,
+                                                         float *__restrict__ x) {
+  constexpr int G = 256 / ZL;
+  const int tid = threadIdx.x, z = tid % ZL, line = blockIdx.x * G + tid / ZL, f = blockIdx.y;
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  if (line >= (int)NYX || z >= g.N_z) return;
+  const int row = line / g.N_x, col = line - row * g.N_x;
+  const float xc = (float)((double)g.x0 + ((double)(float)col + 0.5) * (double)g.delta_xy);
+  const float yc = (float)((double)g.y0 + ((double)(float)row + 0.5) * (double)g.delta_xy);
+  const int nv = proj_num[f];
+  const int *vl = views + (size_t)f * g.max_views;
+  float acc = 0.f;
+  for (int p = 0; p < nv; p++) {
+    const int v = vl[p];
+    const float r = xc * cosv[v] - yc * sinv[v];
+    const float u = (r - g.R0) / g.delta_r - 0.5f;
+    const float fl = floorf(u);
+    const int i0 = (int)fl;
+    const float fr = u - fl;
+    if (i0 < -1 || i0 >= g.N_r) continue;
+    const float a = i0 >= 0 ? q[((size_t)v * g.N_r + i0) * g.N_t + z] : 0.f;
+    const float b = i0 + 1 < g.N_r ? q[((size_t)v * g.N_r + i0 + 1) * g.N_t + z] : 0.f;
+    acc += wv[v] * (a + fr * (b - a));
+  }
+  if (!mask[line]) acc = 0.f;
+  if (g.positivity && acc < 0.f) acc = 0.f;
+  x[((size_t)f * NYX + line) * g.N_z + z] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // qGGMRF surrogate weight b' = b * rho'(delta)/delta (F/XT_ICD_update.c:89-125,
 // 141-168), float.
 __device__ __forceinline__ float xt_qggmrf_ratio(float delta, float p, float c, float sq,

@@ -11,6 +11,7 @@
 // workgroups per frame, each updating one line at a time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,12 +60,12 @@ extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 12) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 13) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 12) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 13) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -215,7 +216,7 @@ struct xt_icd_state {
   TomoInputs *T = nullptr;  // for debug lines only
   // device buffers
   float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *x = nullptr, *Hr = nullptr;
-  float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr;
+  float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr, *fbp_w = nullptr;
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
   int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
   uint8_t *line_mask = nullptr;
@@ -243,6 +244,7 @@ struct xt_icd_state {
 static void free_state(xt_icd_state *s) {
   if (!s) return;
   void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
+                  s->fbp_w,
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
                   s->lines,    s->nhicd_list, s->nhicd_count, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
@@ -496,6 +498,27 @@ static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject 
   XT_HIP(hipMemcpy(s->cosv, S->cosine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->sinv, S->sine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, g.N_p * sizeof(float), hipMemcpyHostToDevice));
+  // backprojection weights: each view's share of [0, pi) among its frame's view
+  // angles taken modulo pi (half the gap to each neighbour, circularly)
+  std::vector<float> wv(g.N_p, 0.f);
+  for (int f = 0; f < g.N_time; f++) {
+    const int n = O->ProjNum[f];
+    std::vector<std::pair<double, int>> a(n);
+    for (int p = 0; p < n; p++) {
+      const int v = O->ProjIdxPtr[f][p];
+      double t = fmod((double)S->ViewPtr[v], XT_PI);
+      if (t < 0) t += XT_PI;
+      a[p] = {t, v};
+    }
+    std::sort(a.begin(), a.end());
+    for (int p = 0; p < n; p++) {
+      const double lo = p > 0 ? a[p - 1].first : a[n - 1].first - XT_PI;
+      const double hi = p + 1 < n ? a[p + 1].first : a[0].first + XT_PI;
+      wv[a[p].second] = (float)(n == 1 ? XT_PI : 0.5 * (hi - lo));
+    }
+  }
+  XT_TRY(dev_alloc(&s->fbp_w, g.N_p));
+  XT_HIP(hipMemcpy(s->fbp_w, wv.data(), g.N_p * sizeof(float), hipMemcpyHostToDevice));
   return XT_OK;
 }
 
@@ -564,6 +587,48 @@ static int init_device(int *dev) {
 static int icd_resync(xt_icd_state *s) {
   return launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 1,
                         nullptr, s->y, s->d, s->e);
+}
+
+template <int ZL>
+static void fbp_bp_launch(xt_icd_state *s, const float *q) {
+  constexpr int G = 256 / ZL;
+  dim3 grid((unsigned)((s->NYX + G - 1) / G), s->g.N_time);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fbp_backproject<ZL>), grid, dim3(256), 0, s->st, s->g, q,
+                     s->fbp_w, s->cosv, s->sinv, s->views, s->proj_num, s->line_mask, s->x);
+}
+
+// x = filtered backprojection of y - d, then e = y - Ax - d.  When the ring
+// offsets are estimated, d first gets a guess from the view-averaged sinogram,
+// so the rings go into d rather than into the starting object.
+static int icd_fbp(xt_icd_state *s) {
+  const XtGeom &g = s->g;
+  float *q = nullptr;
+  XT_TRY(dev_alloc(&q, s->nsino));
+  if (s->update_offset > 0) {
+    const int n = g.N_r * g.N_t;
+    hipLaunchKernelGGL(k_offset_guess_mean, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->st,
+                       g, s->y, q);
+    hipLaunchKernelGGL(k_offset_guess_apply, dim3(1), dim3(1024), 0, s->st, g, q, s->zero_mean,
+                       s->d);
+  }
+  const size_t lds = (size_t)(2 * 4 * g.N_r + (g.N_r + 1) / 2) * sizeof(float);
+  if (lds > 64 * 1024) {
+    (void)hipFree(q);
+    return set_err(XT_E_ARG, "N_r=%d too large for the FBP filter", g.N_r);
+  }
+  hipLaunchKernelGGL(k_fbp_filter, dim3((unsigned)((g.N_t + 3) / 4), g.N_p), dim3(256), lds, s->st,
+                     g, s->y, s->d, q);
+  switch (s->ZL) {
+    case 8: fbp_bp_launch<8>(s, q); break;
+    case 16: fbp_bp_launch<16>(s, q); break;
+    case 32: fbp_bp_launch<32>(s, q); break;
+    default: fbp_bp_launch<64>(s, q); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  (void)hipFree(q);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "FBP: %s", hipGetErrorString(e));
+  return icd_resync(s);
 }
 
 // The A-column table is used when it fits in half of the free device memory
@@ -864,6 +929,13 @@ extern "C" int xt_icd_resync(xt_icd_state *s) {
   return XT_OK;
 }
 
+extern "C" int xt_icd_fbp(xt_icd_state *s) {
+  if (!s) return set_err(XT_E_ARG, "null state");
+  XT_TRY(icd_fbp(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
 extern "C" int xt_icd_synchronize(xt_icd_state *s) {
   if (!s) return set_err(XT_E_ARG, "null state");
   XT_HIP(hipStreamSynchronize(s->st));
@@ -948,6 +1020,14 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   XT_TRY(create_impl(S, O, T, &s));
   dbg(T, "ICD_BackProject: %s, %d masked lines, %d updated at once per frame, z lanes %d, "
       "A table %s\n", xt_version(), s->n_masked, s->conc, s->ZL, s->at_vals ? "on" : "off");
+  if (T->initICD == 0 && g_opt[XT_OPT_INIT_FBP] != 0) {
+    int rf = icd_fbp(s);
+    if (rf != XT_OK) {
+      xt_icd_destroy(s);
+      return rf;
+    }
+    dbg(T, "ICD_BackProject: initial object from filtered backprojection\n");
+  }
   // the error sinogram now lives on the GPU; Projection is released as the
   // reference does (F/XT_ICD_update.c:999)
   xt_multifree(S->Projection, 3);

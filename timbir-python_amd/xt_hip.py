@@ -19,13 +19,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
- OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES) = range(1, 13)
+ OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP) = range(1, 14)
 
 # every symbol include/xt_icd.h declares
 EXPORTS = [
     "xt_last_error", "xt_version", "ICD_BackProject", "xt_set_option", "xt_get_option",
     "xt_icd_create", "xt_icd_iterate", "xt_icd_set_timing", "xt_icd_get_stats", "xt_icd_cost",
-    "xt_icd_resync", "xt_icd_download", "xt_icd_synchronize", "xt_icd_destroy",
+    "xt_icd_resync", "xt_icd_fbp", "xt_icd_download", "xt_icd_synchronize", "xt_icd_destroy",
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
@@ -70,6 +70,7 @@ def lib():
         L.xt_icd_get_stats.argtypes = [P, C.POINTER(IcdStats)]
         L.xt_icd_cost.argtypes = [P, C.POINTER(C.c_double)]
         L.xt_icd_resync.argtypes = [P]
+        L.xt_icd_fbp.argtypes = [P]
         L.xt_icd_download.argtypes = [P, P, P, P]
         L.xt_icd_synchronize.argtypes = [P]
         L.xt_icd_destroy.argtypes = [P]
@@ -232,6 +233,10 @@ class Icd:
 
     def resync(self) -> None:
         _check(lib().xt_icd_resync(self.h))
+
+    def fbp(self) -> None:
+        """Object := filtered backprojection of y - d; e recomputed."""
+        _check(lib().xt_icd_fbp(self.h))
 
     def synchronize(self) -> None:
         _check(lib().xt_icd_synchronize(self.h))
