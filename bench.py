@@ -242,6 +242,12 @@ def main():
         if a.workdir is None:
             shutil.rmtree(work, ignore_errors=True)
     achieved = 12.0 * g["footprint"] / (g["kernel_ms"] * 1e-3) / 1e9 if g["kernel_ms"] > 0 else None
+    # HBM bytes per launch of the voxel-update kernel measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 1 GiB copy
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_icd_lines_slab512x512x32_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf))["traffic_bytes_per_launch"]
     out = {
         "metric": METRIC,
         "value": g["updates"] / g["seconds"],
@@ -261,7 +267,11 @@ def main():
                    "resync": "fp64 every iteration", "prior": "qGGMRF p=1.2 q=2 c=0.01",
                    "offset_update": cfg.update_offset},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-calibrated; profiles/)",
+                     "achieved_basis": "12 B per footprint entry of every performed voxel update "
+                                       "/ summed kernel time (HIP events on the library stream)",
                      "kernel": "k_icd_lines", "kernel_ms_per_step": g["kernel_ms"] / a.steps,
                      "launches_per_step": g["launches"] / a.steps,
                      "bytes_per_step": 12.0 * g["footprint"] / a.steps},
