@@ -88,7 +88,7 @@ def gpu_timed(cfg, path, xt_hip, steps, warmup):
     nl = s1.icd_launches - s0.icd_launches
     vis = s1.line_visits - s0.line_visits
     return dict(seconds=dt, updates=upd, footprint=foot, kernel_ms=kms, launches=nl,
-                iters=steps, visits=vis)
+                iters=steps, visits=vis, lines_in_flight=s1.lines_in_flight)
 
 
 CONVERGED = os.path.join(ROOT, "tests", "golden", "c2_converged_sample.npz")
@@ -189,6 +189,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--workload", choices=("slab", "4d"), default="slab",
+                    help="slab: the 512^2x32 production slab (the metric's workload); 4d: the "
+                         "256^2x16 x 16-frame interlaced reconstruction (timing only)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for the roofline traffic figure)")
@@ -209,7 +212,9 @@ def main():
         sys.exit(2)
     load_pkg()
     from timbir_python_amd import synth, xt_hip
-    cfg = synth.C2
+    cfg = synth.C2 if a.workload == "slab" else synth.C3
+    if a.workload != "slab":
+        a.no_converge = True  # the converged-image reference exists for the slab only
     work = a.workdir or tempfile.mkdtemp(prefix="xt_bench_")
     try:
         log(f"generating {cfg.name}: {cfg.N_xy}^2 x {cfg.N_z}, {cfg.N_p} views, "
@@ -246,7 +251,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 1 GiB copy
     traffic = None
     tf = os.path.join(ROOT, "profiles", "k_icd_lines_slab512x512x32_traffic.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and a.workload == "slab":
         traffic = json.load(open(tf))["traffic_bytes_per_launch"]
     out = {
         "metric": METRIC,
@@ -277,6 +282,7 @@ def main():
                      "bytes_per_step": 12.0 * g["footprint"] / a.steps},
         "voxel_updates_per_step": g["updates"] / a.steps,
         "line_visits_per_step": g["visits"] / a.steps,
+        "lines_in_flight_per_frame": g["lines_in_flight"],
         "time_to_converge": ({
             "criterion": "RMSE <= 1% of the phantom's mean attenuation vs the converged MAP "
                          f"image ({conv['zero']['ref_iters']} GPU ICD iterations from the FBP "

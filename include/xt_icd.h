@@ -89,9 +89,13 @@ enum {
                                 parity's summed update map, so a selected line is updated in
                                 every block of the parity: the same number of voxel updates
                                 in fewer, fuller line visits (default 1) */
-  XT_OPT_INIT_FBP = 13       /* 1: with initICD 0, ICD_BackProject starts from the filtered
+  XT_OPT_INIT_FBP = 13,      /* 1: with initICD 0, ICD_BackProject starts from the filtered
                                 backprojection of y - d instead of zero (no counterpart in
                                 the reference; default 0) */
+  XT_OPT_AUTOTUNE = 14       /* 1: when XT_OPT_LINES is 0, xt_icd_create times one
+                                homogeneous sub-iteration with the automatic number of lines
+                                in flight and with 1/2 and 1/4 of it, restores the state, and
+                                keeps the fastest (default 1) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);
@@ -118,6 +122,7 @@ typedef struct {
   int32_t stopped;
   double line_visits;           /* workgroup visits of a voxel line (a visit updates the line's
                                    slices of one z-block parity) */
+  int32_t lines_in_flight;      /* voxel lines of a frame updated at once (autotuned) */
 } xt_icd_stats;
 
 int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state);

@@ -207,3 +207,26 @@ def test_packed_lanes_match_one_lane_per_slice(scan_root):
     print(f"  packed vs one lane per slice: max rel diff {err:.2e}")
     assert np.abs(vols[0]).max() > 0
     assert err <= 1e-4
+
+
+def test_autotune_restores_state(scan_root):
+    """The launch-shape autotuner runs trial launches at create time and restores
+    the reconstruction state: the FP64 cost right after create is the same with
+    and without it, and the chosen number of lines in flight is a candidate."""
+    cfg = dataclasses.replace(SMALL, iterations=2)
+    path, info = make_scan(cfg, scan_root, "tune")
+    res = {}
+    try:
+        for tune in (0, 1):
+            xt_hip.set_option(xt_hip.OPT_AUTOTUNE, tune)
+            with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
+                c0 = icd.cost()
+                w = icd.stats().lines_in_flight
+                icd.iterate(2)
+                res[tune] = (c0, w, icd.cost())
+    finally:
+        xt_hip.set_option(xt_hip.OPT_AUTOTUNE, 1)
+    print(f"  without autotune {res[0]}, with {res[1]}")
+    assert res[1][0] == res[0][0]
+    assert res[0][1] == 64 and res[1][1] in (16, 32, 64)
+    assert res[1][2] < res[1][0]

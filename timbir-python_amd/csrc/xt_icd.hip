@@ -53,19 +53,19 @@ static int set_err(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 0, 0};  // by XT_OPT_*
+static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0};  // by XT_OPT_*
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 13) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 14) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 13) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 14) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -597,6 +597,73 @@ static void fbp_bp_launch(xt_icd_state *s, const float *q) {
                      s->fbp_w, s->cosv, s->sinv, s->views, s->proj_num, s->line_mask, s->x);
 }
 
+// Launch-shape autotuner: times one homogeneous sub-iteration (its two parity
+// launches) with W = cap, cap/2 and cap/4 workgroups per frame on the real
+// data, restoring the reconstruction state after each trial, and keeps a
+// smaller W only when it is more than 3% faster than the best larger one (the
+// NHICD launches, which cannot be tried before the first iteration, gain from
+// more lines in flight).
+static int icd_autotune(xt_icd_state *s) {
+  std::vector<int> cand;
+  for (int w = s->conc; w >= 16 && cand.size() < 3; w /= 2) cand.push_back(w);
+  if (cand.size() < 2) return XT_OK;
+  const XtGeom &g = s->g;
+  const size_t nmag = (size_t)g.N_time * g.nb * s->NYX;
+  float *e0 = nullptr, *x0 = nullptr, *m0 = nullptr;
+  int rc = dev_alloc(&e0, s->nsino);
+  if (!rc) rc = dev_alloc(&x0, s->nobj);
+  if (!rc) rc = dev_alloc(&m0, nmag);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (!rc && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess))
+    rc = set_err(XT_E_HIP, "autotune events");
+  auto copy = [&](float *dst, const float *src, size_t n) {
+    return hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s->st) == hipSuccess;
+  };
+  if (!rc && !(copy(e0, s->e, s->nsino) && copy(x0, s->x, s->nobj) && copy(m0, s->mag_blk, nmag)))
+    rc = set_err(XT_E_HIP, "autotune snapshot");
+  const int share = s->n_masked / s->K;
+  float best_ms = 1e30f;
+  int best = s->conc;
+  const int saved_timing = s->timing;
+  s->timing = 0;
+  for (size_t i = 0; !rc && i < cand.size(); i++) {
+    s->conc = cand[i];
+    XtIcdLaunch L;
+    L.mode = 0;
+    L.sub_iter = 0;
+    L.K = s->K;
+    L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
+    L.count_stats = 0;
+    L.zorder = 0;
+    if (hipEventRecord(t0, s->st) != hipSuccess) rc = set_err(XT_E_HIP, "autotune record");
+    for (int h = 0; !rc && h < 2; h++) {
+      L.half = h;
+      rc = launch_icd(s, L, share < s->conc ? share : s->conc);
+    }
+    float ms = 0.f;
+    if (!rc && (hipEventRecord(t1, s->st) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+      rc = set_err(XT_E_HIP, "autotune timing");
+    if (!rc && !(copy(s->e, e0, s->nsino) && copy(s->x, x0, s->nobj) && copy(s->mag_blk, m0, nmag)))
+      rc = set_err(XT_E_HIP, "autotune restore");
+    dbg(s->T, "autotune: %d lines in flight per frame: %.3f ms\n", cand[i], ms);
+    if (!rc && ms < best_ms * 0.97f) {
+      best_ms = ms;
+      best = cand[i];
+    }
+  }
+  s->timing = saved_timing;
+  s->launches = 0;
+  s->conc = best;
+  if (!rc && hipStreamSynchronize(s->st) != hipSuccess) rc = set_err(XT_E_HIP, "autotune sync");
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  (void)hipFree(e0);
+  (void)hipFree(x0);
+  (void)hipFree(m0);
+  return rc;
+}
+
 // x = filtered backprojection of y - d, then e = y - Ax - d.  When the ring
 // offsets are estimated, d first gets a guess from the view-averaged sinogram,
 // so the rings go into d rather than into the starting object.
@@ -767,6 +834,8 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = icd_resync(s))) return fail(rc);
   if (hipStreamSynchronize(s->st) != hipSuccess)
     return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
+  if (g_opt[XT_OPT_AUTOTUNE] != 0 && g_opt[XT_OPT_LINES] == 0 && (rc = icd_autotune(s)))
+    return fail(rc);
   *out = s;
   return XT_OK;
 }
@@ -919,6 +988,7 @@ extern "C" int xt_icd_get_stats(xt_icd_state *s, xt_icd_stats *out) {
   out->stop_percentage = s->stop_pct;
   out->stopped = s->stopped;
   out->line_visits = s->total_visits;
+  out->lines_in_flight = s->conc;
   return XT_OK;
 }
 

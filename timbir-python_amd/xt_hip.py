@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
- OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP) = range(1, 14)
+ OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
+ OPT_AUTOTUNE) = range(1, 15)
 
 # every symbol include/xt_icd.h declares
 EXPORTS = [
@@ -42,7 +43,7 @@ class IcdStats(C.Structure):
                 ("footprint_entries", C.c_double), ("icd_kernel_ms", C.c_double),
                 ("icd_launches", C.c_int64), ("last_cost", C.c_double),
                 ("stop_percentage", C.c_double), ("stopped", C.c_int32),
-                ("line_visits", C.c_double)]
+                ("line_visits", C.c_double), ("lines_in_flight", C.c_int32)]
 
 
 Sinogram, ScannedObject, AMatrixCol, TomoInputs = xt_abi.abi_types(False)
