@@ -144,6 +144,17 @@ int xt_icd_download(xt_icd_state *state, Sinogram *S, ScannedObject *O, TomoInpu
 int xt_icd_synchronize(xt_icd_state *state);
 void xt_icd_destroy(xt_icd_state *state);
 
+/* Batches of independent reconstructions (slabs of different sizes, each from
+ * xt_icd_create, all at the same iteration).  xt_icd_batch_iterate advances
+ * every slab by n iterations; from the second iteration on one captured
+ * hipGraph holds each slab's iteration as its own branch, so the slabs overlap
+ * on the GPU.  Results per slab are those of xt_icd_iterate.  The batch does
+ * not own the states.  No counterpart in the reference (one slab per process). */
+typedef struct xt_icd_batch xt_icd_batch;
+int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batch **batch);
+int xt_icd_batch_iterate(xt_icd_batch *batch, int32_t n_iter);
+void xt_icd_batch_destroy(xt_icd_batch *batch);
+
 /* ------------------------------------------------------------------------
  * Engine around the boundary: the reference's command line
  * (F/XT_Init.c:452-629), initStructures (F/XT_Init.c:277-448), the binary

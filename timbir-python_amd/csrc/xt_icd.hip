@@ -916,6 +916,42 @@ static int enqueue_iteration(xt_icd_state *s, bool nhicd) {
   return XT_OK;
 }
 
+// Stop statistic of the iteration, F/XT_ICD_update.c:887-925: sum of |dx| over the
+// lines into scal[4] (graph-capturable).
+static int enqueue_stop_stats(xt_icd_state *s) {
+  int nb = grid_1d((size_t)s->g.N_time * s->NYX, 256, 4096);
+  hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, s->st, s->magsum,
+                     (size_t)s->g.N_time * s->NYX, 0, s->red_wg);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 4);
+  XT_HIP(hipGetLastError());
+  return XT_OK;
+}
+
+static bool resync_due(int Iter) {
+  const int every = (int)g_opt[XT_OPT_RESYNC_EVERY];
+  return every > 0 && Iter % every == 0;
+}
+
+// Host side of the end of an iteration: read the statistics back, decide the
+// stop test, and run what is due and was not already enqueued.
+static int iteration_host(xt_icd_state *s, int Iter, bool resynced) {
+  double st[4], avg = 0;
+  XT_HIP(hipMemcpyAsync(st, s->stats, 4 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipMemcpyAsync(&avg, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  XT_HIP(hipStreamSynchronize(s->st));
+  XT_TRY(collect_timing(s));
+  s->total_updates += st[1];
+  s->total_foot += st[2];
+  s->total_visits += st[3];
+  s->stop_pct = 100.0 * avg / st[0];
+  s->stopped = (s->stop_pct < (double)s->stop_threshold) ? 1 : 0;
+  if (!resynced && resync_due(Iter)) XT_TRY(icd_resync(s));
+  if (g_opt[XT_OPT_COST] != 0) XT_TRY(xt_icd_cost(s, &s->last_cost));
+  dbg(s->T, "ICD iteration %d: updates %.0f in %.0f line visits, average update %.6f%%, cost %.9g\n",
+      Iter, st[1], st[3], s->stop_pct, s->last_cost);
+  return XT_OK;
+}
+
 static int iterate_one(xt_icd_state *s) {
   const int Iter = ++s->iter;
   const bool nhicd = Iter > 1 && !s->no_nhicd;
@@ -945,29 +981,8 @@ static int iterate_one(xt_icd_state *s) {
     XT_TRY(enqueue_iteration(s, nhicd));
   }
   if (s->update_var == 1) XT_TRY(variance_update(s));
-  // stop statistic, F/XT_ICD_update.c:887-925
-  int nb = grid_1d((size_t)s->g.N_time * s->NYX, 256, 4096);
-  hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, s->st, s->magsum,
-                     (size_t)s->g.N_time * s->NYX, 0, s->red_wg);
-  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s->st, s->red_wg, nb, s->scal + 4);
-  XT_HIP(hipGetLastError());
-  double st[4], avg = 0;
-  XT_HIP(hipMemcpyAsync(st, s->stats, 4 * sizeof(double), hipMemcpyDeviceToHost, s->st));
-  XT_HIP(hipMemcpyAsync(&avg, s->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s->st));
-  XT_HIP(hipStreamSynchronize(s->st));
-  XT_TRY(collect_timing(s));
-  s->total_updates += st[1];
-  s->total_foot += st[2];
-  s->total_visits += st[3];
-  s->stop_pct = 100.0 * avg / st[0];
-  s->stopped = (s->stop_pct < (double)s->stop_threshold) ? 1 : 0;
-  int every = (int)g_opt[XT_OPT_RESYNC_EVERY];
-  if (every > 0 && Iter % every == 0) XT_TRY(icd_resync(s));
-  if (g_opt[XT_OPT_COST] != 0) XT_TRY(xt_icd_cost(s, &s->last_cost));
-  dbg(s->T, "ICD iteration %d: updates %.0f in %.0f line visits, average update %.6f%%, cost %.9g\n",
-      Iter, st[1], st[3],
-      s->stop_pct, s->last_cost);
-  return XT_OK;
+  XT_TRY(enqueue_stop_stats(s));
+  return iteration_host(s, Iter, false);
 }
 
 extern "C" int xt_icd_iterate(xt_icd_state *s, int32_t n_iter, int32_t *stop) {
@@ -1047,6 +1062,128 @@ extern "C" int xt_icd_download(xt_icd_state *s, Sinogram *S, ScannedObject *O, T
 }
 
 extern "C" void xt_icd_destroy(xt_icd_state *s) { free_state(s); }
+
+// ---------------------------------------------------------------------------
+// Batches of independent reconstructions (slabs of any size, each created with
+// xt_icd_create).  From the second iteration on, one hipGraph holds every
+// slab's iteration as its own branch (each on its slab's stream, forked from
+// and joined to the batch's stream), so the slabs' launches overlap on the GPU
+// and one graph launch advances the whole batch.
+struct xt_icd_batch {
+  std::vector<xt_icd_state *> s;
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_launches = 0;
+  bool graph_resyncs = false;
+};
+
+extern "C" void xt_icd_batch_destroy(xt_icd_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+extern "C" int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batch **batch) {
+  if (!batch || n <= 0 || !states) return set_err(XT_E_ARG, "bad batch arguments");
+  *batch = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!states[i]) return set_err(XT_E_ARG, "batch slab %d is null", i);
+    if (states[i]->dev != states[0]->dev) return set_err(XT_E_ARG, "batch spans devices");
+    if (states[i]->iter != states[0]->iter)
+      return set_err(XT_E_ARG, "batch slabs at different iterations");
+    for (int k = 0; k < i; k++)
+      if (states[k] == states[i]) return set_err(XT_E_ARG, "slab %d listed twice", i);
+  }
+  xt_icd_batch *b = new xt_icd_batch();
+  b->s.assign(states, states + n);
+  b->join.resize(n, nullptr);
+  bool ok = hipStreamCreate(&b->st) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; ok && i < n; i++)
+    ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    xt_icd_batch_destroy(b);
+    return set_err(XT_E_HIP, "batch streams/events");
+  }
+  *batch = b;
+  return XT_OK;
+}
+
+static int batch_capture(xt_icd_batch *b, const std::vector<int> &iters) {
+  XT_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int rc = XT_OK;
+  hipError_t e = hipEventRecord(b->fork, b->st);
+  b->graph_launches = 0;
+  b->graph_resyncs = (int)g_opt[XT_OPT_RESYNC_EVERY] == 1;
+  for (size_t i = 0; rc == XT_OK && e == hipSuccess && i < b->s.size(); i++) {
+    xt_icd_state *s = b->s[i];
+    e = hipStreamWaitEvent(s->st, b->fork, 0);
+    if (e != hipSuccess) break;
+    const int64_t l0 = s->launches;
+    rc = enqueue_iteration(s, iters[i] > 1 && !s->no_nhicd);
+    if (rc == XT_OK) rc = enqueue_stop_stats(s);
+    if (rc == XT_OK && b->graph_resyncs) rc = icd_resync(s);
+    b->graph_launches += s->launches - l0;
+    s->launches = l0;
+    if (rc == XT_OK) e = hipEventRecord(b->join[i], s->st);
+    if (rc == XT_OK && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->join[i], 0);
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->st, &gr);
+  if (rc != XT_OK || e != hipSuccess || e2 != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc != XT_OK) return rc;
+    return set_err(XT_E_HIP, "batch graph capture: %s",
+                   hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = gr;
+  XT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return XT_OK;
+}
+
+extern "C" int xt_icd_batch_iterate(xt_icd_batch *b, int32_t n_iter) {
+  if (!b) return set_err(XT_E_ARG, "null batch");
+  for (int it = 0; it < n_iter; it++) {
+    std::vector<int> iters(b->s.size());
+    bool graphable = g_opt[XT_OPT_GRAPH] != 0;
+    for (size_t i = 0; i < b->s.size(); i++) {
+      iters[i] = ++b->s[i]->iter;
+      graphable = graphable && iters[i] > 1 && !b->s[i]->timing && b->s[i]->update_var != 1;
+    }
+    if (!graphable) {
+      // first iteration (no NHICD yet) or a slab that needs the host mid-iteration:
+      // every slab's launches on its own stream, overlapping
+      for (size_t i = 0; i < b->s.size(); i++) {
+        xt_icd_state *s = b->s[i];
+        XT_TRY(enqueue_iteration(s, iters[i] > 1 && !s->no_nhicd));
+        if (s->update_var == 1) XT_TRY(variance_update(s));
+        XT_TRY(enqueue_stop_stats(s));
+      }
+      for (size_t i = 0; i < b->s.size(); i++) XT_TRY(iteration_host(b->s[i], iters[i], false));
+      continue;
+    }
+    if (!b->exec) XT_TRY(batch_capture(b, iters));
+    XT_HIP(hipGraphLaunch(b->exec, b->st));
+    XT_HIP(hipEventRecord(b->done, b->st));
+    int64_t per = b->graph_launches / (int64_t)b->s.size();
+    for (size_t i = 0; i < b->s.size(); i++) {
+      XT_HIP(hipStreamWaitEvent(b->s[i]->st, b->done, 0));
+      b->s[i]->launches += per;
+    }
+    for (size_t i = 0; i < b->s.size(); i++)
+      XT_TRY(iteration_host(b->s[i], iters[i], b->graph_resyncs));
+  }
+  return XT_OK;
+}
 
 // ---------------------------------------------------------------------------
 // the drop-in boundary

@@ -205,6 +205,21 @@ def generate(cfg: ScanConfig, path: str, project: Callable[[np.ndarray], np.ndar
 C1 = ScanConfig(name="plain3d_128x128x8", N_xy=128, N_z=8, N_p=256, I0=2e4, iterations=30)
 C2 = ScanConfig(name="slab_512x512x32", N_xy=512, N_z=32, N_p=1024, I0=1e4, iterations=30,
                 num_threads=32)
+def batch_configs(n=24, seed=4):
+    """The batched workload: n independent slabs with image sizes drawn from
+    {128,192,256,384,512}^2 x {4,8,16} slices and view counts from
+    {180,360,720,1024} over [0,180), bins 1.5x the width, Poisson I0=1e4 and 0.3%
+    zingers (seeded draws)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(ScanConfig(name=f"batch{i}", N_xy=int(rng.choice([128, 192, 256, 384, 512])),
+                              N_z=int(rng.choice([4, 8, 16])),
+                              N_p=int(rng.choice([180, 360, 720, 1024])), I0=1e4,
+                              zinger_frac=0.003, num_threads=16, seed=100 + i))
+    return out
+
+
 C3 = ScanConfig(name="interlaced4d_256x256x16x16", N_xy=256, N_z=16, N_p=2048, N_time=16,
                 I0=1e4, interlaced_K=8, interlaced_N_theta=512, motion_per_frame=0.5,
                 sigma_t=2e-4, num_threads=32)

@@ -27,6 +27,7 @@ EXPORTS = [
     "xt_last_error", "xt_version", "ICD_BackProject", "xt_set_option", "xt_get_option",
     "xt_icd_create", "xt_icd_iterate", "xt_icd_set_timing", "xt_icd_get_stats", "xt_icd_cost",
     "xt_icd_resync", "xt_icd_fbp", "xt_icd_download", "xt_icd_synchronize", "xt_icd_destroy",
+    "xt_icd_batch_create", "xt_icd_batch_iterate", "xt_icd_batch_destroy",
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
@@ -72,6 +73,10 @@ def lib():
         L.xt_icd_cost.argtypes = [P, C.POINTER(C.c_double)]
         L.xt_icd_resync.argtypes = [P]
         L.xt_icd_fbp.argtypes = [P]
+        L.xt_icd_batch_create.argtypes = [C.c_int32, P, P]
+        L.xt_icd_batch_iterate.argtypes = [P, C.c_int32]
+        L.xt_icd_batch_destroy.argtypes = [P]
+        L.xt_icd_batch_destroy.restype = None
         L.xt_icd_download.argtypes = [P, P, P, P]
         L.xt_icd_synchronize.argtypes = [P]
         L.xt_icd_destroy.argtypes = [P]
@@ -248,6 +253,30 @@ class Icd:
     def close(self) -> None:
         if self.h:
             lib().xt_icd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class IcdBatch:
+    """xt_icd_batch over open Icd objects (not owned): one graph per iteration."""
+
+    def __init__(self, icds):
+        self.icds = list(icds)
+        arr = (C.c_void_p * len(self.icds))(*[i.h.value for i in self.icds])
+        self.h = C.c_void_p()
+        _check(lib().xt_icd_batch_create(len(self.icds), arr, C.byref(self.h)))
+
+    def iterate(self, n: int = 1) -> None:
+        _check(lib().xt_icd_batch_iterate(self.h, n))
+
+    def close(self) -> None:
+        if self.h:
+            lib().xt_icd_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __enter__(self):
