@@ -1330,6 +1330,13 @@ __global__ void k_vsc_any(int nb, size_t NYX, int n_time, const uint8_t *__restr
   any[(f * 2 + 1) * NYX + c] = a[1];
 }
 
+// x := max(x, 0) (the positivity constraint applied to a starting volume)
+__global__ void k_clamp_nonneg(float *__restrict__ x, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = fmaxf(x[i], 0.f);
+}
+
 // ---------------------------------------------------------------------------
 // Layout changes between the host's [N_time][N_z(+2)][N_y][N_x] object and the
 // device's [N_time][N_y][N_x][N_z].

@@ -22,6 +22,7 @@
 
 #include "../../include/xt_icd.h"
 #include "xt_device.h"
+#include "xt_unet_internal.h"
 
 #ifdef XT_REAL_IS_DOUBLE
 #error "libxt_hip is built with Real_t = float (the reference's production build)"
@@ -32,6 +33,14 @@
 static thread_local char g_err[1024] = "";
 
 static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int xt_report_error(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);
@@ -667,7 +676,7 @@ static int icd_autotune(xt_icd_state *s) {
 // x = filtered backprojection of y - d, then e = y - Ax - d.  When the ring
 // offsets are estimated, d first gets a guess from the view-averaged sinogram,
 // so the rings go into d rather than into the starting object.
-static int icd_fbp(xt_icd_state *s) {
+static int icd_fbp(xt_icd_state *s, bool resync = true) {
   const XtGeom &g = s->g;
   float *q = nullptr;
   XT_TRY(dev_alloc(&q, s->nsino));
@@ -695,7 +704,7 @@ static int icd_fbp(xt_icd_state *s) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->st);
   (void)hipFree(q);
   if (e != hipSuccess) return set_err(XT_E_HIP, "FBP: %s", hipGetErrorString(e));
-  return icd_resync(s);
+  return resync ? icd_resync(s) : XT_OK;
 }
 
 // The A-column table is used when it fits in half of the free device memory
@@ -1017,6 +1026,25 @@ extern "C" int xt_icd_resync(xt_icd_state *s) {
 extern "C" int xt_icd_fbp(xt_icd_state *s) {
   if (!s) return set_err(XT_E_ARG, "null state");
   XT_TRY(icd_fbp(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
+extern "C" int xt_icd_unet_start(xt_icd_state *s, xt_unet *net) {
+  if (!s || !net) return set_err(XT_E_ARG, "null argument");
+  int d0, d1, d2;
+  xt_unet_shape(net, &d0, &d1, &d2);
+  const XtGeom &g = s->g;
+  if (g.N_time != 1 || d0 != g.N_y || d1 != g.N_x || d2 != g.N_z)
+    return set_err(XT_E_ARG, "network volume %dx%dx%d does not match the object %dx%dx%d x %d frames "
+                   "(one frame of N_y x N_x x N_z required)", d0, d1, d2, g.N_y, g.N_x, g.N_z,
+                   g.N_time);
+  XT_TRY(icd_fbp(s, false));
+  XT_TRY(xt_unet_forward_on(net, s->x, s->x, s->st));
+  if (g.positivity)
+    hipLaunchKernelGGL(k_clamp_nonneg, dim3(1024), dim3(256), 0, s->st, s->x, s->nobj);
+  XT_HIP(hipGetLastError());
+  XT_TRY(icd_resync(s));
   XT_HIP(hipStreamSynchronize(s->st));
   return XT_OK;
 }

@@ -32,6 +32,10 @@ EXPORTS = [
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
     "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines",
+    "xt_unet_create", "xt_unet_destroy", "xt_unet_param_count", "xt_unet_get_params",
+    "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
+    "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
+    "xt_unet_forward_host", "xt_unet_loss_grad_host", "xt_unet_train_step_host",
 ]
 
 
@@ -101,6 +105,24 @@ def lib():
         L.xt_vsc_select.argtypes = [P, P, P]
         L.xt_update_lines.argtypes = [P, P, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P,
                                       C.POINTER(C.c_double)]
+        L.xt_unet_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                     C.c_int32, C.POINTER(P)]
+        L.xt_unet_destroy.argtypes = [P]
+        L.xt_unet_destroy.restype = None
+        L.xt_unet_param_count.argtypes = [P]
+        L.xt_unet_param_count.restype = C.c_int64
+        L.xt_unet_get_params.argtypes = [P, P, C.c_int64]
+        L.xt_unet_set_params.argtypes = [P, P, C.c_int64]
+        L.xt_unet_get_scale.argtypes = [P, C.POINTER(C.c_float)]
+        L.xt_unet_forward.argtypes = [P, P, P]
+        L.xt_unet_loss_grad.argtypes = [P, P, P, C.POINTER(C.c_double), P]
+        L.xt_unet_adam_step.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.xt_unet_train_step.argtypes = [P, P, P, C.c_float, C.POINTER(C.c_double)]
+        L.xt_unet_last_ms.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.xt_icd_unet_start.argtypes = [P, P]
+        L.xt_unet_forward_host.argtypes = [P, P, P]
+        L.xt_unet_loss_grad_host.argtypes = [P, P, P, C.POINTER(C.c_double), P]
+        L.xt_unet_train_step_host.argtypes = [P, P, P, C.c_float, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -284,6 +306,100 @@ class IcdBatch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# -- learned warm start -----------------------------------------------------
+class UNet:
+    """xt_unet: the warm-start 3D U-Net for volumes [d0][d1][d2] (d2 fastest).
+
+    Volumes are host float32 arrays, copied through the network's own device
+    staging buffers; every call returns after its work is done."""
+
+    def __init__(self, shape, scale=1000.0, seed=0, train=False):
+        self.shape = tuple(int(d) for d in shape)
+        self.h = C.c_void_p()
+        _check(lib().xt_unet_create(*self.shape, float(scale), int(seed), 1 if train else 0,
+                                    C.byref(self.h)))
+        self.n = int(lib().xt_unet_param_count(self.h))
+
+    @property
+    def scale(self) -> float:
+        s = C.c_float()
+        _check(lib().xt_unet_get_scale(self.h, C.byref(s)))
+        return s.value
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n, np.float32)
+        _check(lib().xt_unet_get_params(self.h, p.ctypes.data, self.n))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n:
+            raise XtError(f"{p.size} parameters given, network has {self.n}")
+        _check(lib().xt_unet_set_params(self.h, p.ctypes.data, self.n))
+
+    def _vol(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        if x.shape != self.shape:
+            raise XtError(f"volume of shape {x.shape}, network expects {self.shape}")
+        return x
+
+    def forward(self, x) -> np.ndarray:
+        x = self._vol(x)
+        out = np.empty_like(x)
+        _check(lib().xt_unet_forward_host(self.h, x.ctypes.data, out.ctypes.data))
+        return out
+
+    def loss_grad(self, x, tgt, want_grad=False):
+        x, tgt = self._vol(x), self._vol(tgt)
+        loss = C.c_double()
+        g = np.zeros(self.n, np.float32) if want_grad else None
+        _check(lib().xt_unet_loss_grad_host(self.h, x.ctypes.data, tgt.ctypes.data,
+                                            C.byref(loss), g.ctypes.data if want_grad else None))
+        return loss.value, g
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
+        _check(lib().xt_unet_adam_step(self.h, lr, beta1, beta2, eps))
+
+    def train_step(self, x, tgt, lr) -> float:
+        x, tgt = self._vol(x), self._vol(tgt)
+        loss = C.c_double()
+        _check(lib().xt_unet_train_step_host(self.h, x.ctypes.data, tgt.ctypes.data, lr,
+                                             C.byref(loss)))
+        return loss.value
+
+    def last_ms(self):
+        f, b = C.c_double(), C.c_double()
+        _check(lib().xt_unet_last_ms(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def save(self, path) -> None:
+        np.savez(path, params=self.get_params(), scale=np.float32(self.scale),
+                 arch=np.array("unet3d-4lvl-16-32-64-128"))
+
+    @classmethod
+    def load(cls, path, shape, train=False):
+        z = np.load(path, allow_pickle=False)
+        net = cls(shape, scale=float(z["scale"]), train=train)
+        net.set_params(z["params"])
+        return net
+
+    def close(self) -> None:
+        if self.h:
+            lib().xt_unet_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def icd_unet_start(icd: "Icd", net: UNet) -> None:
+    """Warm start in place: FBP, the network on the FBP volume, resync of e."""
+    _check(lib().xt_icd_unet_start(icd.h, net.h))
 
 
 # -- seams ------------------------------------------------------------------
