@@ -1,5 +1,5 @@
 """CPU tests of the host side of libxt_hip.so (no GPU calls): it loads and
-exports every include/xt_icd.h symbol, its engine set-up reproduces the
+exports every include/xt_icd.h and include/xt_unet.h symbol, its engine set-up reproduces the
 reference's argsParser + initStructures (checked against the oracle's own
 restatement), its pointer trees have the reference layout, and bad input is an
 error return rather than a process exit."""
@@ -17,8 +17,10 @@ from timbir_python_amd import synth, xt_hip
 
 
 def test_library_exports_every_declared_symbol():
-    header = open(os.path.join(ROOT, "include", "xt_icd.h")).read()
-    declared = set(re.findall(r"\b(ICD_BackProject|xt_[a-z0-9_]+)\s*\(", header))
+    declared = set()
+    for h in ("xt_icd.h", "xt_unet.h"):
+        header = open(os.path.join(ROOT, "include", h)).read()
+        declared |= set(re.findall(r"\b(ICD_BackProject|xt_[a-z0-9_]+)\s*\(", header))
     assert declared == set(xt_hip.EXPORTS)
     lib = xt_hip.lib()
     for name in declared:
