@@ -95,9 +95,13 @@ CONVERGED = os.path.join(ROOT, "tests", "golden", "c2_converged_sample.npz")
 CPU_ICD = os.path.join(ROOT, "tests", "golden", "c2_cpu_icd_sample.npz")
 
 
+WEIGHTS = os.path.join(PKG, "weights", "unet_warmstart_256x256x16.npz")
+
+
 def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
-    """GPU ICD one iteration at a time from a zero start or from the filtered
-    backprojection: wall time (FBP included; the RMSE evaluations between
+    """GPU ICD one iteration at a time from a zero start (warm "zero"), from the
+    filtered backprojection ("fbp") or from the warm-start network applied to
+    it ("unet"): wall time (start included; the RMSE evaluations between
     iterations are not timed) until the volume is within tol RMSE, relative to
     the phantom's mean attenuation, of the converged MAP image (sampled voxels of
     the reference file).  Also reports the RMSE to the CPU ICD after its 30
@@ -109,13 +113,22 @@ def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
     cpu = np.load(CPU_ICD) if os.path.exists(CPU_ICD) else None
     xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
     times, rmse, rmse_cpu = [], [], []
+    net = None
+    if warm == "unet":
+        if not os.path.exists(WEIGHTS):
+            return None
+        net = xt_hip.UNet.load(WEIGHTS, (cfg.N_xy, cfg.N_xy, cfg.N_z))
     with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
         icd.synchronize()
         t0 = time.perf_counter()
-        if warm:
+        if warm == "fbp":
             icd.fbp()
+        elif warm == "unet":
+            xt_hip.icd_unet_start(icd, net)
         icd.synchronize()
         t_init = time.perf_counter() - t0
+        if net is not None:
+            net.close()
         for _ in range(max_iter):
             icd.synchronize()
             t0 = time.perf_counter()
@@ -129,7 +142,7 @@ def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
                 rmse_cpu.append(float(np.sqrt(np.mean((v[cpu["idx"]] - cpu["values"]) ** 2)) /
                                       float(cpu["mean_att"])))
             if len(times) % 10 == 0:
-                log(f"{'fbp' if warm else 'zero'} start, iteration {len(times)}: "
+                log(f"{warm} start, iteration {len(times)}: "
                     f"RMSE to the converged image {rmse[-1]:.3%}")
             if rmse[-1] <= tol:
                 break
@@ -232,6 +245,13 @@ def cpu_sample(cfg, path, target_s):
         o.free(S, O, T)
 
 
+def _ratio(conv, slow, fast):
+    a, b = conv.get(slow), conv.get(fast)
+    if not a or not b or not a["seconds"] or not b["seconds"]:
+        return None
+    return a["seconds"] / b["seconds"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,15 +306,15 @@ def main():
         conv = None
         if not a.no_converge:
             conv = {}
-            for warm in (1, 0):
+            for warm in ("unet", "fbp", "zero"):
                 c = gpu_converge(cfg, work, xt_hip, a.converge_iters, warm)
                 if c is None:
-                    log(f"no converged reference at {CONVERGED}: time to converge not measured")
-                    conv = None
-                    break
-                conv["fbp" if warm else "zero"] = c
-                log(f"{'FBP' if warm else 'zero'} start: within 1% RMSE of the converged image "
+                    log(f"{warm} start not measured (no converged reference or weights)")
+                    continue
+                conv[warm] = c
+                log(f"{warm} start: within 1% RMSE of the converged image "
                     f"after {c['iters']} iterations, {c['seconds']} s")
+            conv = conv if "fbp" in conv else None
         cpu = None
         if not a.no_cpu:
             cpu = cpu_sample(cfg, work, a.cpu_seconds)
@@ -342,8 +362,13 @@ def main():
         "lines_in_flight_per_frame": g["lines_in_flight"],
         "time_to_converge": ({
             "criterion": "RMSE <= 1% of the phantom's mean attenuation vs the converged MAP "
-                         f"image ({conv['zero']['ref_iters']} GPU ICD iterations from the FBP "
+                         f"image ({conv['fbp']['ref_iters']} GPU ICD iterations from the FBP "
                          "start, the lowest MAP cost reached; 262144-voxel sample)",
+            "starts": "zero: plain ICD; fbp: filtered backprojection with ring-offset guess; "
+                      "unet: the warm-start 3D U-Net (BF16 MFMA, weights in "
+                      "timbir-python_amd/weights/) on the FBP volume, inference timed",
+            "speedup_unet_vs_fbp": _ratio(conv, "fbp", "unet"),
+            "speedup_unet_vs_zero": _ratio(conv, "zero", "unet"),
             **{k: {"seconds": c["seconds"], "iterations": c["iters"],
                    "init_seconds": round(c["init_seconds"], 4),
                    "rmse_by_iteration": [round(r, 5) for r in c["rmse"]],

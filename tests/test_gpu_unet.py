@@ -120,3 +120,50 @@ def test_icd_unet_start_is_network_on_fbp(scan_root):
             icd.resync()
             assert abs(icd.cost() - c_start) <= 1e-9 * abs(c_start)
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-7 * np.abs(want).max())
+
+
+def test_trained_network_gains_over_fbp_start(scan_root):
+    """The committed weights (trained on 256^2x16, 720-view pairs) on a held-out
+    128^2x16 scan with the same view density: the network's start is closer to
+    the converged image than the FBP start, and stays closer after 3 ICD
+    iterations."""
+    import os
+    import dataclasses
+    from timbir_python_amd import synth
+    weights = os.path.join(os.path.dirname(xt_hip.__file__), "weights",
+                           "unet_warmstart_256x256x16.npz")
+    cfg = dataclasses.replace(synth.C1, name="unet_gain", N_xy=128, N_z=16, N_p=360, I0=2e4,
+                              zinger_frac=0.005, num_threads=16, seed=424242)
+    from timbir_python_amd import warmstart
+    path = os.path.join(str(scan_root), "unet_gain")
+    synth.generate(cfg, path, warmstart.gpu_projector(cfg, path))
+    shape = (cfg.N_xy, cfg.N_xy, cfg.N_z)
+    xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
+    print("[gpu] unet gain: reference ICD from FBP, 80 iterations", flush=True)
+    with xt_hip.Engine(path, cfg.cli_args()) as eng:
+        with xt_hip.Icd(eng) as icd:
+            icd.fbp()
+            icd.iterate(80)
+            icd.download()
+        ref = eng.volume()[0]
+        mean_att = float(ref[ref > 2.5e-4].mean())
+        curves = {}
+        with xt_hip.UNet.load(weights, shape) as net:
+            for start in ("fbp", "unet"):
+                with xt_hip.Icd(eng) as icd:
+                    if start == "fbp":
+                        icd.fbp()
+                    else:
+                        xt_hip.icd_unet_start(icd, net)
+                    c = []
+                    for it in range(4):
+                        if it:
+                            icd.iterate(1)
+                        icd.download()
+                        c.append(rel(eng.volume()[0], ref) * np.linalg.norm(ref) /
+                                 np.sqrt(ref.size) / mean_att)
+                    curves[start] = c
+    print("  RMSE to the converged image after 0..3 iterations: "
+          + "; ".join(f"{k} " + " ".join(f"{v:.2%}" for v in c) for k, c in curves.items()))
+    assert curves["unet"][0] < 0.5 * curves["fbp"][0]
+    assert curves["unet"][3] < curves["fbp"][3]
