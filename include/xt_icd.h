@@ -92,10 +92,17 @@ enum {
   XT_OPT_INIT_FBP = 13,      /* 1: with initICD 0, ICD_BackProject starts from the filtered
                                 backprojection of y - d instead of zero (no counterpart in
                                 the reference; default 0) */
-  XT_OPT_AUTOTUNE = 14       /* 1: when XT_OPT_LINES is 0, xt_icd_create times one
+  XT_OPT_AUTOTUNE = 14,      /* 1: when XT_OPT_LINES is 0, xt_icd_create times one
                                 homogeneous sub-iteration with the automatic number of lines
                                 in flight and with 1/2 and 1/4 of it, restores the state, and
                                 keeps the fastest (default 1) */
+  XT_OPT_BATCH_FUSE = 15     /* 1: xt_icd_batch_iterate runs each voxel-update pass of all
+                                slabs as one launch per kernel variant (slabs of the same
+                                line width), the slabs' own small kernels between them;
+                                0: each slab's iteration as its own graph branch (default 0:
+                                one slab's lines in flight already fill the workgroup slots
+                                the line kernel's registers allow, so the fused passes run the
+                                slabs nearly in turn, with a barrier per pass) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);

@@ -1,7 +1,8 @@
 """Batches of independent slabs of different sizes advanced together, one
-hipGraph (one branch per slab) per iteration: each slab's result is the one its
-own xt_icd_iterate gives.  With one line in flight per frame every slab's run is
-deterministic, so the comparison is exact."""
+hipGraph per iteration -- fused (each voxel-update pass of all slabs in one
+launch per kernel variant) or one branch per slab: each slab's result is the
+one its own xt_icd_iterate gives.  With one line in flight per frame every
+slab's run is deterministic, so the comparison is exact."""
 import dataclasses
 
 import numpy as np
@@ -17,9 +18,12 @@ SLABS = [dataclasses.replace(TINY, iterations=4),
          dataclasses.replace(TINY, name="tiny48", N_xy=48, N_p=64, seed=21, iterations=4)]
 
 
-def test_batch_matches_single_slabs(scan_root):
+@pytest.mark.parametrize("fuse", [1, 0], ids=["fused", "branches"])
+def test_batch_matches_single_slabs(scan_root, fuse):
     paths = [make_scan(c, scan_root, "batch")[0] for c in SLABS]
-    saved = {o: xt_hip.get_option(o) for o in (xt_hip.OPT_LINES, xt_hip.OPT_RESYNC_EVERY)}
+    saved = {o: xt_hip.get_option(o)
+             for o in (xt_hip.OPT_LINES, xt_hip.OPT_RESYNC_EVERY, xt_hip.OPT_BATCH_FUSE)}
+    xt_hip.set_option(xt_hip.OPT_BATCH_FUSE, fuse)
     xt_hip.set_option(xt_hip.OPT_LINES, 1)
     xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
     try:
@@ -34,7 +38,7 @@ def test_batch_matches_single_slabs(scan_root):
         icds = [xt_hip.Icd(e) for e in engs]
         try:
             with xt_hip.IcdBatch(icds) as b:
-                print(f"[gpu] batch of {len(icds)} slabs", flush=True)
+                print(f"[gpu] batch of {len(icds)} slabs, fuse={fuse}", flush=True)
                 b.iterate(1)
                 b.iterate(3)
             batched = []

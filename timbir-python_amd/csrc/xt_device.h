@@ -569,9 +569,13 @@ __device__ __forceinline__ int xt_permute(int i, int n, uint32_t key) {
 //     wait for them before the next line, whose loads then see them.
 // Lines updated concurrently by different workgroups read each other's e and
 // x as they happen to be (a bounded Jacobi coupling; the concurrency is capped
-// on the host).
+// on the host).  The body serves one slab (k_icd_lines: bx = blockIdx.x of gx
+// workgroups per frame, frame f = blockIdx.y) or one slab of a batch
+// (k_icd_lines_multi).
 template <int ZL, bool TAB, bool PACK>
-__global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+__device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffers &B,
+                                               const XtIcdLaunch &L, const int bx, const int gx,
+                                               const int f) {
   constexpr int NT = 256;
   constexpr int ZP = PACK ? ZL / 2 : ZL;  // lanes per view group
   constexpr int G = NT / ZP;              // view groups
@@ -592,7 +596,6 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   __shared__ int s_cnt_col, s_perf;
 
   const int tid = threadIdx.x;
-  const int f = blockIdx.y;
   const int nv = B.proj_num[f];
   const int *vlist = B.views + (size_t)f * g.max_views;
   const size_t NYX = (size_t)g.N_y * g.N_x;
@@ -650,14 +653,14 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
   const int plane = g.pm ? par * g.N_p * g.N_r * g.H : 0;
   const int row_len = g.pm ? g.H : g.N_t;
 
-  // homogeneous lines: positions blockIdx.x, +gridDim.x, ... of the random order;
-  // NHICD lines: a contiguous run of the raster-ordered list per workgroup (the
+  // homogeneous lines: positions bx, bx + gx, ... of the random order; NHICD
+  // lines: a contiguous run of the raster-ordered list per workgroup (the
   // reference walks its selection lists in raster order, F/XT_NHICD.c:189-199)
-  int i0 = lo + (int)blockIdx.x, i1 = hi, istep = (int)gridDim.x;
+  int i0 = lo + bx, i1 = hi, istep = gx;
   if (L.mode == 1) {
     const long long len = hi - lo;
-    i0 = lo + (int)(len * blockIdx.x / gridDim.x);
-    i1 = lo + (int)(len * (blockIdx.x + 1) / gridDim.x);
+    i0 = lo + (int)(len * bx / gx);
+    i1 = lo + (int)(len * (bx + 1) / gx);
     istep = 1;
   }
   for (int i = i0; i < i1; i += istep) {
@@ -946,6 +949,37 @@ __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtI
     atomicAdd(&B.stats[2], n_foot);
     atomicAdd(&B.stats[3], n_vis);
   }
+}
+
+template <int ZL, bool TAB, bool PACK>
+__global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+  icd_lines_body<ZL, TAB, PACK>(g, B, L, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
+}
+
+// One slab of a batch launch: its state, its launch parameters, its workgroups
+// per frame gx and its first row y0 of the grid (rows y0 .. y0 + N_time - 1).
+struct XtIcdSlab {
+  XtGeom g;
+  XtIcdBuffers B;
+  XtIcdLaunch L;
+  int gx;
+  int y0;
+};
+
+// Every slab of a batch in one launch: grid = (max gx, sum of frames); the
+// workgroups beyond a slab's own gx return at once.  Slabs are listed by
+// increasing y0.
+template <int ZL, bool TAB, bool PACK>
+__global__ void __launch_bounds__(256) k_icd_lines_multi(const XtIcdSlab *__restrict__ slabs,
+                                                         int n) {
+  int i = 0;
+  while (i + 1 < n && (int)blockIdx.y >= slabs[i + 1].y0) i++;
+  const int gx = slabs[i].gx;
+  if ((int)blockIdx.x >= gx) return;
+  const XtGeom g = slabs[i].g;
+  const XtIcdBuffers B = slabs[i].B;
+  const XtIcdLaunch L = slabs[i].L;
+  icd_lines_body<ZL, TAB, PACK>(g, B, L, (int)blockIdx.x, gx, (int)blockIdx.y - slabs[i].y0);
 }
 
 // Line selection per (frame, z-block parity): the update magnitude summed over

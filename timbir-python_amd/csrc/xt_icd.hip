@@ -69,12 +69,12 @@ extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 14) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 15) return set_err(XT_E_ARG, "unknown option %d", option);
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 14) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 15) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -1106,12 +1106,172 @@ struct xt_icd_batch {
   hipGraphExec_t exec = nullptr;
   int64_t graph_launches = 0;
   bool graph_resyncs = false;
+  // fused form: slabs grouped by kernel variant; per (pass, group) a table of
+  // XtIcdSlab entries in device memory
+  bool fused = false;
+  struct Group {
+    int ZL = 0;
+    bool tab = false, pack = false;
+    std::vector<int> slabs;
+    int rows = 0;
+    size_t lds = 0;
+  };
+  std::vector<Group> groups;
+  struct Pass {
+    size_t off = 0;  // first entry in tab
+    int n = 0, gx_max = 0;
+  };
+  std::vector<Pass> passes;  // [pass][group], pass = sub_iter * 4 + (nhicd ? 2 : 0) + half
+  XtIcdSlab *tab = nullptr;
 };
+
+// Groups of slabs that share a kernel variant, and the per-pass tables of the
+// fused form.  Slabs with different sub-iteration counts or NHICD settings keep
+// the branch form.
+static int batch_plan(xt_icd_batch *b) {
+  b->fused = false;
+  if (g_opt[XT_OPT_BATCH_FUSE] == 0) return XT_OK;
+  const xt_icd_state *s0 = b->s[0];
+  for (xt_icd_state *s : b->s)
+    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd) return XT_OK;
+  for (size_t i = 0; i < b->s.size(); i++) {
+    xt_icd_state *s = b->s[i];
+    XtIcdLaunch L0{};
+    const bool tab = s->at_vals != nullptr, pack = icd_packable(s->g, s->ZL, L0);
+    size_t k = 0;
+    while (k < b->groups.size() &&
+           !(b->groups[k].ZL == s->ZL && b->groups[k].tab == tab && b->groups[k].pack == pack))
+      k++;
+    if (k == b->groups.size()) {
+      b->groups.emplace_back();
+      b->groups[k].ZL = s->ZL;
+      b->groups[k].tab = tab;
+      b->groups[k].pack = pack;
+    }
+    auto &G = b->groups[k];
+    G.slabs.push_back((int)i);
+    G.rows += s->g.N_time;
+    G.lds = std::max(G.lds, icd_lds_bytes(s->g, s->ZL, tab));
+  }
+  const int K = s0->K, ng = (int)b->groups.size();
+  std::vector<XtIcdSlab> host;
+  b->passes.assign((size_t)K * 4 * ng, xt_icd_batch::Pass());
+  for (int j = 0; j < K; j++)
+    for (int m = 0; m < 2; m++)
+      for (int h = 0; h < 2; h++)
+        for (int k = 0; k < ng; k++) {
+          auto &P = b->passes[((size_t)j * 4 + m * 2 + h) * ng + k];
+          P.off = host.size();
+          int y0 = 0;
+          for (int i : b->groups[k].slabs) {
+            xt_icd_state *s = b->s[i];
+            XtIcdSlab e;
+            e.g = s->g;
+            e.B = buffers_of(s);
+            e.L.mode = m;
+            e.L.half = h;
+            e.L.sub_iter = j;
+            e.L.K = K;
+            e.L.seed = (uint32_t)(int64_t)g_opt[XT_OPT_SEED];
+            e.L.count_stats = m == 0;
+            e.L.zorder = 0;
+            const int per = s->n_masked / K;
+            const int share = (j == K - 1) ? s->n_masked - j * per : per;
+            e.gx = std::max(1, m == 0 ? std::min(share, s->conc) : s->conc);
+            e.y0 = y0;
+            y0 += s->g.N_time;
+            P.gx_max = std::max(P.gx_max, e.gx);
+            host.push_back(e);
+          }
+          P.n = (int)(host.size() - P.off);
+        }
+  XT_HIP(hipMalloc((void **)&b->tab, host.size() * sizeof(XtIcdSlab)));
+  XT_HIP(hipMemcpy(b->tab, host.data(), host.size() * sizeof(XtIcdSlab), hipMemcpyHostToDevice));
+  b->fused = true;
+  return XT_OK;
+}
+
+template <int ZL>
+static void fused_launch(hipStream_t st, const xt_icd_batch::Group &G,
+                         const xt_icd_batch::Pass &P, const XtIcdSlab *tab) {
+  const dim3 grid(P.gx_max, G.rows), block(256);
+  const XtIcdSlab *t = tab + P.off;
+  if (G.tab) {
+    if (G.pack)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, true>), grid, block, G.lds,
+                         st, t, P.n);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, false>), grid, block, G.lds,
+                         st, t, P.n);
+  } else {
+    if (G.pack)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, true>), grid, block, G.lds,
+                         st, t, P.n);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, false>), grid, block,
+                         G.lds, st, t, P.n);
+  }
+}
+
+static int fused_pass(xt_icd_batch *b, int pass) {
+  const int ng = (int)b->groups.size();
+  for (int k = 0; k < ng; k++) {
+    const auto &G = b->groups[k];
+    const auto &P = b->passes[(size_t)pass * ng + k];
+    switch (G.ZL) {
+      case 8: fused_launch<8>(b->st, G, P, b->tab); break;
+      case 16: fused_launch<16>(b->st, G, P, b->tab); break;
+      case 32: fused_launch<32>(b->st, G, P, b->tab); break;
+      default: fused_launch<64>(b->st, G, P, b->tab); break;
+    }
+    XT_HIP(hipGetLastError());
+  }
+  for (xt_icd_state *s : b->s) s->launches++;
+  return XT_OK;
+}
+
+// One iteration of every slab on the batch stream: the voxel-update passes
+// fused across slabs, each slab's ring-offset updates and NHICD selection
+// between them, the stop statistic and (resync) the FP64 resync at the end.
+// The slabs' streams are pointed at the batch stream meanwhile.
+static int enqueue_batch_fused(xt_icd_batch *b, bool nhicd, bool resync) {
+  std::vector<hipStream_t> own;
+  for (xt_icd_state *s : b->s) {
+    own.push_back(s->st);
+    s->st = b->st;
+  }
+  int rc = XT_OK;
+  for (xt_icd_state *s : b->s)
+    hipLaunchKernelGGL(k_begin_iteration, dim3(1), dim3(64), 0, b->st, s->iter_dev, s->stats, 8);
+  const int K = b->s[0]->K;
+  for (int j = 0; rc == XT_OK && j < K; j++) {
+    for (int h = 0; rc == XT_OK && h < 2; h++) {
+      rc = fused_pass(b, j * 4 + h);
+      for (xt_icd_state *s : b->s)
+        if (rc == XT_OK && s->update_offset > 1) rc = offset_update(s);
+    }
+    if (!nhicd) continue;
+    for (xt_icd_state *s : b->s)
+      if (rc == XT_OK) rc = vsc_select(s);
+    for (int h = 0; rc == XT_OK && h < 2; h++) {
+      rc = fused_pass(b, j * 4 + 2 + h);
+      for (xt_icd_state *s : b->s)
+        if (rc == XT_OK && s->update_offset > 1) rc = offset_update(s);
+    }
+  }
+  for (xt_icd_state *s : b->s) {
+    if (rc == XT_OK) rc = enqueue_stop_stats(s);
+    if (rc == XT_OK && resync) rc = icd_resync(s);
+  }
+  for (size_t i = 0; i < b->s.size(); i++) b->s[i]->st = own[i];
+  return rc;
+}
 
 extern "C" void xt_icd_batch_destroy(xt_icd_batch *b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->tab) (void)hipFree(b->tab);
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->done) (void)hipEventDestroy(b->done);
@@ -1142,6 +1302,11 @@ extern "C" int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batc
     xt_icd_batch_destroy(b);
     return set_err(XT_E_HIP, "batch streams/events");
   }
+  int rc = batch_plan(b);
+  if (rc != XT_OK) {
+    xt_icd_batch_destroy(b);
+    return rc;
+  }
   *batch = b;
   return XT_OK;
 }
@@ -1149,10 +1314,21 @@ extern "C" int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batc
 static int batch_capture(xt_icd_batch *b, const std::vector<int> &iters) {
   XT_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
   int rc = XT_OK;
-  hipError_t e = hipEventRecord(b->fork, b->st);
+  hipError_t e = hipSuccess;
   b->graph_launches = 0;
   b->graph_resyncs = (int)g_opt[XT_OPT_RESYNC_EVERY] == 1;
-  for (size_t i = 0; rc == XT_OK && e == hipSuccess && i < b->s.size(); i++) {
+  if (b->fused) {
+    std::vector<int64_t> l0;
+    for (xt_icd_state *s : b->s) l0.push_back(s->launches);
+    rc = enqueue_batch_fused(b, iters[0] > 1 && !b->s[0]->no_nhicd, b->graph_resyncs);
+    for (size_t i = 0; i < b->s.size(); i++) {
+      b->graph_launches += b->s[i]->launches - l0[i];
+      b->s[i]->launches = l0[i];
+    }
+  } else {
+    e = hipEventRecord(b->fork, b->st);
+  }
+  for (size_t i = 0; !b->fused && rc == XT_OK && e == hipSuccess && i < b->s.size(); i++) {
     xt_icd_state *s = b->s[i];
     e = hipStreamWaitEvent(s->st, b->fork, 0);
     if (e != hipSuccess) break;
@@ -1187,9 +1363,24 @@ extern "C" int xt_icd_batch_iterate(xt_icd_batch *b, int32_t n_iter) {
       iters[i] = ++b->s[i]->iter;
       graphable = graphable && iters[i] > 1 && !b->s[i]->timing && b->s[i]->update_var != 1;
     }
+    // the batch stream starts after whatever the slabs' own streams hold
+    for (size_t i = 0; i < b->s.size(); i++) {
+      XT_HIP(hipEventRecord(b->join[i], b->s[i]->st));
+      XT_HIP(hipStreamWaitEvent(b->st, b->join[i], 0));
+    }
+    bool fusable = b->fused;
+    for (xt_icd_state *s : b->s) fusable = fusable && !s->timing && s->update_var != 1;
+    if (!graphable && fusable) {
+      // first iteration (no NHICD yet), eager
+      XT_TRY(enqueue_batch_fused(b, iters[0] > 1 && !b->s[0]->no_nhicd, false));
+      XT_HIP(hipEventRecord(b->done, b->st));
+      for (xt_icd_state *s : b->s) XT_HIP(hipStreamWaitEvent(s->st, b->done, 0));
+      for (size_t i = 0; i < b->s.size(); i++) XT_TRY(iteration_host(b->s[i], iters[i], false));
+      continue;
+    }
     if (!graphable) {
-      // first iteration (no NHICD yet) or a slab that needs the host mid-iteration:
-      // every slab's launches on its own stream, overlapping
+      // a slab that needs the host mid-iteration (or the branch form's first
+      // iteration): every slab's launches on its own stream, overlapping
       for (size_t i = 0; i < b->s.size(); i++) {
         xt_icd_state *s = b->s[i];
         XT_TRY(enqueue_iteration(s, iters[i] > 1 && !s->no_nhicd));

@@ -20,9 +20,9 @@ LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
- OPT_AUTOTUNE) = range(1, 15)
+ OPT_AUTOTUNE, OPT_BATCH_FUSE) = range(1, 16)
 
-# every symbol include/xt_icd.h declares
+# every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
     "xt_last_error", "xt_version", "ICD_BackProject", "xt_set_option", "xt_get_option",
     "xt_icd_create", "xt_icd_iterate", "xt_icd_set_timing", "xt_icd_get_stats", "xt_icd_cost",
