@@ -96,12 +96,16 @@ CPU_ICD = os.path.join(ROOT, "tests", "golden", "c2_cpu_icd_sample.npz")
 
 
 WEIGHTS = os.path.join(PKG, "weights", "unet_warmstart_256x256x16.npz")
+MULTIRES_ITERS = (20, 10)  # ICD iterations of the 128^2 and 256^2 stages before the 512^2 one
 
 
 def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
     """GPU ICD one iteration at a time from a zero start (warm "zero"), from the
-    filtered backprojection ("fbp") or from the warm-start network applied to
-    it ("unet"): wall time (start included; the RMSE evaluations between
+    filtered backprojection ("fbp"), from the warm-start network applied to
+    it ("unet") or from the reference launcher's coarser stages ("multires":
+    128^2 and 256^2 reconstructions of the same sinogram, each stage upsampled
+    into the next, their set-up and iterations timed): wall time (start
+    included; the RMSE evaluations between
     iterations are not timed) until the volume is within tol RMSE, relative to
     the phantom's mean attenuation, of the converged MAP image (sampled voxels of
     the reference file).  Also reports the RMSE to the CPU ICD after its 30
@@ -125,6 +129,10 @@ def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
             icd.fbp()
         elif warm == "unet":
             xt_hip.icd_unet_start(icd, net)
+        elif warm == "multires":
+            from timbir_python_amd import multires
+            vol, off = multires.coarse_stages(path, cfg, MULTIRES_ITERS)
+            multires.start_from_coarse(icd, vol, off)
         icd.synchronize()
         t_init = time.perf_counter() - t0
         if net is not None:
@@ -306,7 +314,7 @@ def main():
         conv = None
         if not a.no_converge:
             conv = {}
-            for warm in ("unet", "fbp", "zero"):
+            for warm in ("unet", "fbp", "multires", "zero"):
                 c = gpu_converge(cfg, work, xt_hip, a.converge_iters, warm)
                 if c is None:
                     log(f"{warm} start not measured (no converged reference or weights)")
@@ -366,7 +374,11 @@ def main():
                          "start, the lowest MAP cost reached; 262144-voxel sample)",
             "starts": "zero: plain ICD; fbp: filtered backprojection with ring-offset guess; "
                       "unet: the warm-start 3D U-Net (BF16 MFMA, weights in "
-                      "timbir-python_amd/weights/) on the FBP volume, inference timed",
+                      "timbir-python_amd/weights/) on the FBP volume, inference timed; "
+                      f"multires: the reference launcher's stages ({MULTIRES_ITERS[0]} ICD "
+                      f"iterations at 128^2, {MULTIRES_ITERS[1]} at 256^2, initICD-2 "
+                      "upsampling between them), stage set-up and iterations timed",
+            "speedup_unet_vs_multires": _ratio(conv, "multires", "unet"),
             "speedup_unet_vs_fbp": _ratio(conv, "fbp", "unet"),
             "speedup_unet_vs_zero": _ratio(conv, "zero", "unet"),
             **{k: {"seconds": c["seconds"], "iterations": c["iters"],

@@ -138,6 +138,11 @@ int xt_icd_set_timing(xt_icd_state *state, int32_t on);
 int xt_icd_get_stats(xt_icd_state *state, xt_icd_stats *stats);
 int xt_icd_cost(xt_icd_state *state, double *cost);          /* FP64 MAP cost now */
 int xt_icd_resync(xt_icd_state *state);                       /* FP64 e = y - Ax - d now */
+/* The inverse of xt_icd_download: the object (slices 1..N_z of every frame) and
+ * ProjOffset of the structs become the state's x and d, and e = y - Ax - d is
+ * recomputed in FP64 (a stepwise host's restart from a volume of its own, e.g.
+ * the previous stage of a multi-resolution run). */
+int xt_icd_upload(xt_icd_state *state, Sinogram *S, ScannedObject *O, TomoInputs *T);
 /* Replace the object by the filtered backprojection of y - d (per frame: ramp
  * filter of sample spacing delta_r after a 3-point median along the bins that
  * removes isolated zingers; backprojection with the A-matrix geometry and the
@@ -182,6 +187,15 @@ void xt_multifree(void *r, int d);
 
 /* ------------------------------------------------------------------------
  * Test seams. */
+
+/* initObject's multi-resolution upsampling (Source_Code_4D_Fast/XT_ICD_update.c:
+ * upsample_object_bilinear_2D :355-388 for initICD 2, upsample_object_bilinear_3D
+ * :389-435 for initICD 3): the coarse [Nzc][Nyc][Nxc] object doubled along y, x
+ * (fine [Nzc][2Nyc][2Nxc]) or along z, y, x (fine [2Nzc][2Nyc][2Nxc]) on the GPU,
+ * bit-exact with the reference's float/double steps.  ICD_BackProject uses it for
+ * --initICD 2/3 (the previous stage's object_n<r>_time_<i>.bin at half size). */
+int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
+                       const Real_t *coarse, Real_t *fine);
 
 /* DetectorResponseProfile, F/XT_Profile.c:128-217: H_r[N_p][65]. */
 int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r);

@@ -108,6 +108,8 @@ class Oracle:
         L.oracle_parse_view_info.argtypes = [C.c_char_p, C.c_int32, P, P]
         L.oracle_parse_view_info.restype = C.c_int32
         L.oracle_bin_views.argtypes = [P, C.c_int32, C.c_int32, self.R, self.R, P, P]
+        L.oracle_upsample_object.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]
+        L.oracle_upsample_object.restype = None
         L.oracle_interlaced_views.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                               C.c_double, P, P, P, P, P]
         L.oracle_interlaced_views.restype = C.c_int32
@@ -116,6 +118,15 @@ class Oracle:
         L.oracle_sample_updates.restype = C.c_int64
 
     # -- engine -----------------------------------------------------------
+    def upsample_object(self, init_icd, coarse):
+        """initObject's upsampling (initICD 2: y, x; 3: z, y, x) of a coarse
+        [Nzc][Nyc][Nxc] volume."""
+        c = np.ascontiguousarray(coarse, self.np_real)
+        nz, ny, nx = c.shape
+        out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), self.np_real)
+        self.lib.oracle_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data)
+        return out
+
     def setup(self, path, args):
         S, O, T = self.Sinogram(), self.ScannedObject(), self.TomoInputs()
         n, arr = argv_array(args)

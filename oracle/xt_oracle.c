@@ -13,6 +13,7 @@
 #include "xt_oracle.h"
 
 #include <ctype.h>
+#include <stddef.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -1492,4 +1493,57 @@ int32_t oracle_engine_run(const char *dir, int32_t argc, char **argv, int32_t fl
   oracle_engine_free(&S, &O, &T);
   if (T.debug_file_ptr) fclose(T.debug_file_ptr);
   return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-resolution upsampling, Source_Code_4D_Fast/XT_ICD_update.c:316-435.
+ * Along one axis of n coarse samples c[0..n-1] the 2n fine samples are
+ *   f[0] = c[0], f[1] = (3 c[0] + c[1]) / 4, f[2n-2] = (c[n-2] + 3 c[n-1]) / 4,
+ *   f[2n-1] = c[n-1], and for 0 < k < n-1: f[2k] = (c[k-1] + 3 c[k]) / 4,
+ *   f[2k+1] = (3 c[k] + c[k+1]) / 4.
+ * Every weighted sum is formed in double (the reference's 3.0 literals) and
+ * rounded to Real_t once, except the interior odd rows of the y pass, whose
+ * reference text multiplies by the integer 3 and so sums in Real_t before the
+ * division by 4.0 (:339,:411). */
+static Real_t up_even(Real_t prev, Real_t cur) { return (Real_t)((prev + 3.0 * cur) / 4.0); }
+static Real_t up_odd(Real_t cur, Real_t next) { return (Real_t)((3.0 * cur + next) / 4.0); }
+static Real_t up_odd_real(Real_t cur, Real_t next) {
+  Real_t t = (Real_t)3 * cur;
+  t = t + next;
+  return (Real_t)(t / 4.0);
+}
+
+/* one axis: n coarse samples at stride `cs` -> 2n fine samples at stride `fs`;
+ * `real_odd` selects the Real_t form of the interior odd samples */
+static void up_axis(const Real_t *c, ptrdiff_t cs, int32_t n, Real_t *f, ptrdiff_t fs, int real_odd) {
+  f[0] = c[0];
+  f[fs] = up_odd(c[0], c[cs]);
+  f[(2 * n - 1) * fs] = c[(n - 1) * cs];
+  f[(2 * n - 2) * fs] = up_even(c[(n - 2) * cs], c[(n - 1) * cs]);
+  for (int32_t k = 1; k < n - 1; k++) {
+    f[2 * k * fs] = up_even(c[(k - 1) * cs], c[k * cs]);
+    f[(2 * k + 1) * fs] =
+        real_odd ? up_odd_real(c[k * cs], c[(k + 1) * cs]) : up_odd(c[k * cs], c[(k + 1) * cs]);
+  }
+}
+
+void oracle_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
+                            const Real_t *coarse, Real_t *fine) {
+  const int32_t Nx = 2 * Nxc, Ny = 2 * Nyc;
+  Real_t *bx = malloc(sizeof(Real_t) * (size_t)Nzc * Nyc * Nx);
+  Real_t *by = init_icd == 3 ? malloc(sizeof(Real_t) * (size_t)Nzc * Ny * Nx) : fine;
+  for (int32_t z = 0; z < Nzc; z++)
+    for (int32_t j = 0; j < Nyc; j++)  /* x pass */
+      up_axis(coarse + ((size_t)z * Nyc + j) * Nxc, 1, Nxc, bx + ((size_t)z * Nyc + j) * Nx, 1, 0);
+  for (int32_t z = 0; z < Nzc; z++)
+    for (int32_t k = 0; k < Nx; k++)  /* y pass */
+      up_axis(bx + (size_t)z * Nyc * Nx + k, Nx, Nyc, by + (size_t)z * Ny * Nx + k, Nx, 1);
+  if (init_icd == 3) {
+    for (int32_t j = 0; j < Ny; j++)  /* z pass */
+      for (int32_t k = 0; k < Nx; k++)
+        up_axis(by + (size_t)j * Nx + k, (ptrdiff_t)Ny * Nx, Nzc, fine + (size_t)j * Nx + k,
+                (ptrdiff_t)Ny * Nx, 0);
+    free(by);
+  }
+  free(bx);
 }

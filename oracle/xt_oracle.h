@@ -21,6 +21,8 @@
  *   oracle_vsc_select            bit-exact vs the reference's XT_NHICD.c (oracle/_ref)
  *   oracle_parse_view_info       the reference's view_info.txt fixtures
  *   oracle_interlaced_views      golden log Source_Code_4D_Fast/info.txt:1-175
+ *   oracle_upsample_object       closed forms: constants and linear ramps reproduced
+ *                                exactly (tests/test_oracle_pins.py)
  *   voxel update / surrogate     closed forms: quadratic prior + no zingers reduces ICD to
  *                                Gauss-Seidel on (A'WA+H)x = A'W(y-d), solved densely;
  *                                qGGMRF derivative vs finite differences of the value
@@ -102,6 +104,15 @@ int32_t oracle_parse_view_info(const char *path, int32_t max_views, Real_t *view
 /* Frame binning; proj_idx_flat is [N_time][N_p] row-major, proj_num [N_time]. */
 void oracle_bin_views(const Real_t *times, int32_t N_p, int32_t N_time, Real_t Rtime0,
                       Real_t delta_Rtime, int32_t *proj_idx_flat, int32_t *proj_num);
+/* Multi-resolution initial object (initObject with initICD 2 or 3,
+ * Source_Code_4D_Fast/XT_ICD_update.c:316-435,484-512): the coarse
+ * [Nzc][Nyc][Nxc] reconstruction upsampled 2x along y and x (init_icd 2; fine
+ * [Nzc][2Nyc][2Nxc]) or along z, y and x (init_icd 3; fine [2Nzc][2Nyc][2Nxc]),
+ * with the reference's mix of double and Real_t arithmetic.  Each coarse size
+ * that is upsampled must be >= 2. */
+void oracle_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
+                            const Real_t *coarse, Real_t *fine);
+
 /* Interlaced bit-reversed order and clipping (4D_Recon_Launcher_for_Real_Data/
  * XT_Interlaced_Angles.py:6-27,30-34,107-139).  Returns the kept count; the
  * deleted views/times go to del_views/del_times, their count to *n_del. */

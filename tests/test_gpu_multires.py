@@ -1,0 +1,75 @@
+"""Multi-resolution initialisation (initICD 2/3, F/XT_ICD_update.c:316-512) on the
+GPU: the upsampling bit-exact against the C restatement in oracle/xt_oracle.c
+(pinned by closed forms in tests/test_oracle_pins.py), ICD_BackProject's
+initICD 2 path through the run folder, and the stage loop as a warm start."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import py_oracle
+from conftest import SMALL, make_scan
+from timbir_python_amd import multires, xt_hip
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("init_icd", [2, 3])
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 5, 7), (8, 64, 48)], ids=str)
+def test_upsample_bitexact(init_icd, shape):
+    print(f"[gpu] upsample initICD {init_icd} {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape) + init_icd)
+    c = rng.uniform(0.0, 2e-3, shape).astype(np.float32)
+    got = xt_hip.upsample_object(init_icd, c)
+    want = py_oracle.Oracle(False).upsample_object(init_icd, c)
+    assert got.shape == want.shape
+    np.testing.assert_array_equal(got, want)
+
+
+def test_upsample_rejects_small_sizes():
+    with pytest.raises(xt_hip.XtError):
+        xt_hip.upsample_object(2, np.zeros((2, 1, 4), np.float32))
+
+
+def test_icd_backproject_initicd2(scan_root):
+    """Coarse stage (voxels twice the size) through the engine, then the fine
+    stage with --initICD 2 and no iterations: its object is the coarse object
+    upsampled exactly as the reference's initObject does."""
+    cfg = SMALL
+    path, _ = make_scan(cfg, scan_root, "multires")
+    coarse = multires.stage_configs(cfg, 2)[0]
+    print("[gpu] initICD 2 through the engine", flush=True)
+    xt_hip.engine_run(path, coarse.cli_args(iterations=3))
+    c = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float32).reshape(
+        cfg.N_z, cfg.N_xy // 2, cfg.N_xy // 2)
+    assert np.abs(c).max() > 0
+    xt_hip.engine_run(path, cfg.cli_args(iterations=0, init_icd=2))
+    f = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float32).reshape(
+        cfg.N_z, cfg.N_xy, cfg.N_xy)
+    np.testing.assert_array_equal(f, py_oracle.Oracle(False).upsample_object(2, c))
+
+
+def test_multires_start_beats_zero_start(scan_root):
+    """Two coarse stages then the final one: after 3 final-resolution iterations
+    the volume is closer to the converged image than 3 iterations from zero."""
+    cfg = dataclasses.replace(SMALL, N_xy=64, seed=12)
+    path, _ = make_scan(cfg, scan_root, "mrstart")
+    xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
+    with xt_hip.Engine(path, cfg.cli_args()) as eng:
+        with xt_hip.Icd(eng) as icd:
+            icd.fbp()
+            icd.iterate(60)
+            icd.download()
+        ref = eng.volume()[0].copy()
+    vol, off = multires.coarse_stages(path, cfg, [10, 5])
+    err = {}
+    for start in ("zero", "multires"):
+        with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
+            if start == "multires":
+                multires.start_from_coarse(icd, vol, off)
+            icd.iterate(3)
+            icd.download()
+            err[start] = float(np.sqrt(np.mean((eng.volume()[0] - ref) ** 2)))
+    print(f"  RMSE to the converged image after 3 iterations: {err}")
+    assert err["multires"] < 0.5 * err["zero"]

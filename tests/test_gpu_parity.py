@@ -10,6 +10,7 @@ Tolerances (written per test):
   1e-4 relative    per-voxel updates (FP32 theta sums in a different order)
 """
 import ctypes as C
+import dataclasses
 import os
 
 import numpy as np
@@ -20,6 +21,11 @@ from timbir_python_amd import xt_abi, xt_hip
 import py_oracle
 
 pytestmark = pytest.mark.gpu
+
+# voxels 3 and 6 detector bins wide (the coarse stages of a multi-resolution run):
+# A columns of up to 8 and 14 bins, the line kernel's wide-column variant
+WIDE3 = dataclasses.replace(TINY, name="wide3", N_xy=16, mult_xy=3.0)
+WIDE6 = dataclasses.replace(TINY, name="wide6", N_xy=8, mult_xy=6.0)
 
 
 def _print_case(name, cfg):
@@ -58,8 +64,8 @@ def test_detector_response_bitexact(cfg, scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
-def test_amatrix_columns_bitexact(scan_root, oracle32):
-    cfg = TINY
+@pytest.mark.parametrize("cfg", [TINY, WIDE3, WIDE6], ids=lambda c: c.name)
+def test_amatrix_columns_bitexact(cfg, scan_root, oracle32):
     _print_case("amatrix_columns", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     try:
@@ -73,10 +79,10 @@ def test_amatrix_columns_bitexact(scan_root, oracle32):
         rows[:S.N_p] = 0
         cols[:S.N_p] = O.N_x - 1
         views[:S.N_p] = np.arange(S.N_p)
-        cnt, idx, val = xt_hip.amatrix_columns(eng, H, rows, cols, views)
+        cnt, idx, val = xt_hip.amatrix_columns(eng, H, rows, cols, views, max_entries=16)
         AM = oracle32.AMatrixCol
-        vals = np.zeros(8, np.float32)
-        inds = np.zeros(8, np.int32)
+        vals = np.zeros(16, np.float32)
+        inds = np.zeros(16, np.int32)
         a = AM()
         a.values = vals.ctypes.data
         a.index = inds.ctypes.data
@@ -93,7 +99,7 @@ def test_amatrix_columns_bitexact(scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
-@pytest.mark.parametrize("cfg", [TINY, SMALL4D], ids=lambda c: c.name)
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6], ids=lambda c: c.name)
 def test_forward_project(cfg, scan_root, oracle32, oracle64a):
     """Ax within 1e-5 relative of the float reference system matrix (FP64 sums).
     The reference's double build uses a different detector table (its float
@@ -243,10 +249,10 @@ def test_vsc_select_bitexact(scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
-@pytest.mark.parametrize("atable", [1, 0], ids=["atable", "onthefly"])
+@pytest.mark.parametrize("cfg,atable", [(TINY, 1), (TINY, 0), (WIDE3, 0), (WIDE6, 0)],
+                         ids=["atable", "onthefly", "wide3", "wide6"])
 @pytest.mark.parametrize("Iter", [1, 2])
-def test_update_lines_per_voxel(Iter, atable, scan_root, oracle32):
-    cfg = TINY
+def test_update_lines_per_voxel(Iter, cfg, atable, scan_root, oracle32):
     _print_case(f"update_lines Iter={Iter} atable={atable}", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     xt_hip.set_option(xt_hip.OPT_ATABLE, atable)

@@ -446,3 +446,56 @@ def test_cost_closed_form(oracle64, scan_root):
         assert c == pytest.approx(expect, rel=1e-12)
     finally:
         oracle64.free(S, O, T)
+
+
+# ---- multi-resolution upsampling (initObject, F/XT_ICD_update.c:316-435) ----
+def _ramp(shape, a, coef):
+    idx = np.indices(shape).astype(np.float64)
+    return a + sum(c * i for c, i in zip(coef, idx))
+
+
+@pytest.mark.parametrize("init_icd", [2, 3])
+def test_upsample_closed_forms(init_icd):
+    """Constants stay exact; a ramp a + b*z + c*y + d*x with dyadic coefficients
+    comes back as the ramp at the fine positions k -/+ 1/4 (interior) and clamped
+    at the ends -- every weight and index of the three passes."""
+    o = py_oracle.Oracle(False)
+    nz, ny, nx = 3, 4, 5
+    const = np.full((nz, ny, nx), 0.75, np.float32)
+    np.testing.assert_array_equal(o.upsample_object(init_icd, const), 0.75)
+    coef = (0.5, 0.25, 2.0)
+    ramp = _ramp((nz, ny, nx), 1.0, coef).astype(np.float32)
+    got = o.upsample_object(init_icd, ramp).astype(np.float64)
+
+    def pos(n):  # fine index -> coarse coordinate, clamped at the ends
+        p = np.arange(2 * n) / 2.0 - 0.25
+        return np.clip(p, 0.0, n - 1.0)
+    pz = pos(nz) if init_icd == 3 else np.arange(nz, dtype=np.float64)
+    want = (1.0 + coef[0] * pz[:, None, None] + coef[1] * pos(ny)[None, :, None] +
+            coef[2] * pos(nx)[None, None, :])
+    np.testing.assert_array_equal(got, want)
+
+
+def test_upsample_rounding_paths():
+    """The interior odd rows of the y pass sum in float (the reference's integer
+    3), every other sample in double: on random floats the two paths differ in
+    the last bit somewhere, and the oracle follows each where the text says."""
+    o = py_oracle.Oracle(False)
+    rng = np.random.default_rng(5)
+    c = rng.uniform(0.0, 2e-3, (2, 16, 16)).astype(np.float32)
+    got = o.upsample_object(2, c)
+    # x pass in double, as the reference
+    bx = np.zeros((2, 16, 32), np.float32)
+    c64 = c.astype(np.float64)
+    bx[..., 0], bx[..., -1] = c[..., 0], c[..., -1]
+    bx[..., 1] = ((3.0 * c64[..., 0] + c64[..., 1]) / 4.0).astype(np.float32)
+    bx[..., -2] = ((c64[..., -2] + 3.0 * c64[..., -1]) / 4.0).astype(np.float32)
+    bx[..., 2:-2:2] = ((c64[..., :-2] + 3.0 * c64[..., 1:-1]) / 4.0)[..., :14].astype(np.float32)
+    bx[..., 3:-2:2] = ((3.0 * c64[..., 1:-1] + c64[..., 2:]) / 4.0)[..., :14].astype(np.float32)
+    b64 = bx.astype(np.float64)
+    odd_real = ((np.float32(3) * bx[:, 1:-1] + bx[:, 2:]) / np.float32(4)).astype(np.float32)
+    odd_double = ((3.0 * b64[:, 1:-1] + b64[:, 2:]) / 4.0).astype(np.float32)
+    np.testing.assert_array_equal(got[:, 3:-2:2], odd_real[:, :14])
+    assert (odd_real != odd_double).any()  # the distinction is real on these inputs
+    even = ((b64[:, :-2] + 3.0 * b64[:, 1:-1]) / 4.0).astype(np.float32)
+    np.testing.assert_array_equal(got[:, 2:-2:2], even[:, :14])

@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #define XT_MAXA 6          // max detector bins of one A column (2*delta_xy/delta_r + 2)
+#define XT_MAXA_WIDE 14    // the same for voxels up to 6 detector bins wide (coarse stages)
 #define XT_NBINS 64        // DETECTOR_RESPONSE_BINS
 #define XT_HR_STRIDE 65
 #define XT_FP_BINS 64      // detector bins per forward-projection workgroup
@@ -44,6 +45,7 @@ struct XtGeom {
   // z-block parity, [2][N_p][N_r][H] with H = N_t/2, detector row t in plane
   // ((t / bs) & 1) at position (t / (2 bs)) * bs + t % bs (bs slices per block)
   int pm, bs, H;
+  int wide;  // A columns of more than XT_MAXA bins (voxels wider than 2 detector bins)
 };
 
 // Offset of sinogram entry (view p, bin r, row t) in the HBM layout.
@@ -55,11 +57,12 @@ __device__ __forceinline__ size_t xt_sidx(const XtGeom &g, int p, int r, int t) 
 
 // ---------------------------------------------------------------------------
 // System-matrix column for (row, col, view): the exact float steps of
-// F/XT_AMatrix.c:50-110.  Writes the first candidate bin and XT_MAXA values
+// F/XT_AMatrix.c:50-110.  Writes the first candidate bin and MA values
 // (0 for bins that are not part of the column); returns the entry count.
+template <int MA>
 __device__ __forceinline__ int xt_amatrix_column(const XtGeom &g, const float *__restrict__ H,
                                                  float cosv, float sinv, int row, int col,
-                                                 int *first, float vals[XT_MAXA]) {
+                                                 int *first, float (&vals)[MA]) {
 #pragma clang fp contract(off)
   float x = (float)((double)g.x0 + ((double)(float)col + 0.5) * (double)g.delta_xy);
   float y = (float)((double)g.y0 + ((double)(float)row + 0.5) * (double)g.delta_xy);
@@ -67,7 +70,7 @@ __device__ __forceinline__ int xt_amatrix_column(const XtGeom &g, const float *_
   float rmin = r - g.delta_xy;
   float rmax = r + g.delta_xy;
 #pragma unroll
-  for (int b = 0; b < XT_MAXA; b++) vals[b] = 0.f;
+  for (int b = 0; b < MA; b++) vals[b] = 0.f;
   *first = 0;
   if (rmax < g.R0 || rmin > g.RMax) return 0;
   int imin = (int)floorf((rmin - g.R0) / g.delta_r);
@@ -77,7 +80,7 @@ __device__ __forceinline__ int xt_amatrix_column(const XtGeom &g, const float *_
   *first = imin;
   int count = 0;
 #pragma unroll
-  for (int b = 0; b < XT_MAXA; b++) {
+  for (int b = 0; b < MA; b++) {
     int j = imin + b;
     if (j <= imax) {
       float Rc = (float)((double)g.R0 + ((double)(float)j + 0.5) * (double)g.delta_r);
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(256) k_detector_response(XtGeom g, const float
 // column can reach the chunk; each voxel's exact A column is built once, and
 // its entries times the line's slices are added to the accumulators.
 // mode 0: out64 = Ax.   mode 1: e = y - Ax - d (float).
-template <int ZL>
+template <int ZL, int MA>
 __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *__restrict__ x,
                                                          const float *__restrict__ Hr,
                                                          const float *__restrict__ cosv,
@@ -182,7 +185,7 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
                                                          float *__restrict__ e) {
   constexpr int G = 256 / ZL;
   __shared__ double acc[XT_FP_BINS * ZL];
-  __shared__ float cval[G][ZL][XT_MAXA];
+  __shared__ float cval[G][ZL][MA];
   __shared__ int cfirst[G][ZL];
   const int v = blockIdx.y, j0 = blockIdx.x * XT_FP_BINS;
   const int tid = threadIdx.x, zl = tid % ZL, grp = tid / ZL;
@@ -228,16 +231,16 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
         {
           const int k = kb + zl;
           int first = 0;
-          float vals[XT_MAXA];
+          float vals[MA];
 #pragma unroll
-          for (int b = 0; b < XT_MAXA; b++) vals[b] = 0.f;
+          for (int b = 0; b < MA; b++) vals[b] = 0.f;
           if (k <= k1) {
             const int row = by_row ? o : k, col = by_row ? k : o;
             xt_amatrix_column(g, H, c, s, row, col, &first, vals);
           }
           cfirst[grp][zl] = first;
 #pragma unroll
-          for (int b = 0; b < XT_MAXA; b++) cval[grp][zl][b] = vals[b];
+          for (int b = 0; b < MA; b++) cval[grp][zl][b] = vals[b];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -250,7 +253,7 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
             const double xv = (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl];
             const int first = cfirst[grp][m];
 #pragma unroll
-            for (int b = 0; b < XT_MAXA; b++) {
+            for (int b = 0; b < MA; b++) {
               const float a = cval[grp][m][b];
               const int jj = first + b - j0;
               if (a != 0.f && jj >= 0 && jj < XT_FP_BINS) atomicAdd(&acc[jj * ZL + zl], xv * (double)a);
@@ -572,20 +575,23 @@ __device__ __forceinline__ int xt_permute(int i, int n, uint32_t key) {
 // on the host).  The body serves one slab (k_icd_lines: bx = blockIdx.x of gx
 // workgroups per frame, frame f = blockIdx.y) or one slab of a batch
 // (k_icd_lines_multi).
-template <int ZL, bool TAB, bool PACK>
+template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA>
 __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffers &B,
                                                const XtIcdLaunch &L, const int bx, const int gx,
                                                const int f) {
   constexpr int NT = 256;
+  constexpr bool WIDE = !TAB && MA > XT_MAXA;  // columns of more than 6 bins, in LDS as floats
   constexpr int ZP = PACK ? ZL / 2 : ZL;  // lanes per view group
   constexpr int G = NT / ZP;              // view groups
-  constexpr int U = TAB ? 4 : 2;          // views in flight per group and step
-  constexpr int NA = TAB ? 4 : XT_MAXA;   // bins a column can hold
+  constexpr int U = TAB ? 4 : (WIDE ? 1 : 2);  // views in flight per group and step
+  constexpr int NA = TAB ? 4 : MA;        // bins a column can hold
   constexpr uint32_t OOB = 0x80000000u;   // buffer offset that reads 0 / drops the atomic
   extern __shared__ float4 smem4[];
   float4 *a_v4 = smem4;                                            // [max_views] bins 0..3
   float2 *a_v2 = (float2 *)(a_v4 + g.max_views);                   // [max_views] bins 4..5
-  int *a_base = (int *)(a_v2 + (TAB ? 0 : g.max_views));           // [max_views] e offset
+  float *a_w = (float *)smem4;                                     // WIDE: [max_views][MA]
+  int *a_base = WIDE ? (int *)(a_w + (size_t)g.max_views * MA)
+                     : (int *)(a_v2 + (TAB ? 0 : g.max_views));    // [max_views] e offset
   float *red1 = (float *)(a_base + g.max_views);                   // [NT]
   float *red2 = red1 + NT;                                         // [NT]
   float *xn = red2 + NT;                                           // [11][ZL]
@@ -707,15 +713,20 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
       for (int p = tid; p < nv; p += NT) {
         const int v = vlist[p];
         int first = 0;
-        float vals[XT_MAXA];
+        float vals[MA];
         xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx, &first,
                           vals);
-        a_v4[p] = make_float4(vals[0] * g.delta_t, vals[1] * g.delta_t, vals[2] * g.delta_t,
-                              vals[3] * g.delta_t);
-        a_v2[p] = make_float2(vals[4] * g.delta_t, vals[5] * g.delta_t);
+        if constexpr (WIDE) {
+#pragma unroll
+          for (int b = 0; b < MA; b++) a_w[(size_t)p * MA + b] = vals[b] * g.delta_t;
+        } else {
+          a_v4[p] = make_float4(vals[0] * g.delta_t, vals[1] * g.delta_t, vals[2] * g.delta_t,
+                                vals[3] * g.delta_t);
+          a_v2[p] = make_float2(vals[4] * g.delta_t, vals[5] * g.delta_t);
+        }
         a_base[p] = plane + (v * g.N_r + first) * row_len;
 #pragma unroll
-        for (int b = 0; b < XT_MAXA; b++) cnt_local += vals[b] > 0.f;
+        for (int b = 0; b < MA; b++) cnt_local += vals[b] > 0.f;
       }
     }
     for (int q = tid; q < 11 * ZL; q += NT) {
@@ -759,16 +770,21 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
           if (q < nv) {
             int p = q + rot;
             if (p >= nv) p -= nv;
-            const float4 a4 = a_v4[p];
             off[u] = (uint32_t)a_base[p] * sizeof(float) + lane_off;
-            av[u][0] = a4.x;
-            av[u][1] = a4.y;
-            av[u][2] = a4.z;
-            av[u][3] = a4.w;
-            if constexpr (!TAB) {
-              const float2 a2 = a_v2[p];
-              av[u][4] = a2.x;
-              av[u][5] = a2.y;
+            if constexpr (WIDE) {
+#pragma unroll
+              for (int b = 0; b < NA; b++) av[u][b] = a_w[(size_t)p * MA + b];
+            } else {
+              const float4 a4 = a_v4[p];
+              av[u][0] = a4.x;
+              av[u][1] = a4.y;
+              av[u][2] = a4.z;
+              av[u][3] = a4.w;
+              if constexpr (!TAB) {
+                const float2 a2 = a_v2[p];
+                av[u][4] = a2.x;
+                av[u][5] = a2.y;
+              }
             }
           }
         }
@@ -911,12 +927,21 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
         int p = q + rot;
         if (p >= nv) p -= nv;
         const uint32_t o = (uint32_t)a_base[p] * sizeof(float) + lane_off;
-        const float4 a4 = a_v4[p];
-        float av[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, 0.f, 0.f};
-        if constexpr (!TAB) {
-          const float2 a2 = a_v2[p];
-          av[4] = a2.x;
-          av[5] = a2.y;
+        float av[NA > XT_MAXA ? NA : XT_MAXA] = {};
+        if constexpr (WIDE) {
+#pragma unroll
+          for (int b = 0; b < NA; b++) av[b] = a_w[(size_t)p * MA + b];
+        } else {
+          const float4 a4 = a_v4[p];
+          av[0] = a4.x;
+          av[1] = a4.y;
+          av[2] = a4.z;
+          av[3] = a4.w;
+          if constexpr (!TAB) {
+            const float2 a2 = a_v2[p];
+            av[4] = a2.x;
+            av[5] = a2.y;
+          }
         }
 #pragma unroll
         for (int b = 0; b < NA; b++)
@@ -951,9 +976,9 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
   }
 }
 
-template <int ZL, bool TAB, bool PACK>
+template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA>
 __global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
-  icd_lines_body<ZL, TAB, PACK>(g, B, L, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
+  icd_lines_body<ZL, TAB, PACK, MA>(g, B, L, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
 }
 
 // One slab of a batch launch: its state, its launch parameters, its workgroups
@@ -1362,6 +1387,57 @@ __global__ void k_vsc_any(int nb, size_t NYX, int n_time, const uint8_t *__restr
   for (int b = 0; b < nb; b++) a[b & 1] |= blk[(f * nb + b) * NYX + c];
   any[(f * 2 + 0) * NYX + c] = a[0];
   any[(f * 2 + 1) * NYX + c] = a[1];
+}
+
+// Multi-resolution upsampling (initObject with initICD 2/3,
+// F/XT_ICD_update.c:316-435): one axis of a [A][n][B] array, n coarse samples
+// at stride B, to [A][2n][B].  Along the axis f[0] = c[0], f[1] = (3c0 + c1)/4,
+// f[2n-2] = (c[n-2] + 3c[n-1])/4, f[2n-1] = c[n-1], interior f[2k] =
+// (c[k-1] + 3c[k])/4, f[2k+1] = (3c[k] + c[k+1])/4; each sum in double and
+// rounded to float once, except the interior odd samples of the y pass
+// (real_odd), which the reference sums in float (its integer 3, :339,:411).
+__device__ __forceinline__ float up_even(float p, float c) {
+  return (float)(((double)p + 3.0 * (double)c) / 4.0);
+}
+__device__ __forceinline__ float up_odd(float c, float n) {
+  return (float)((3.0 * (double)c + (double)n) / 4.0);
+}
+__device__ __forceinline__ float up_odd_real(float c, float n) {
+#pragma clang fp contract(off)
+  // two roundings, as the reference's float expression (an FMA would round once)
+  const float t = 3.f * c;
+  return (t + n) * 0.25f;
+}
+
+__global__ void k_upsample_axis(const float *__restrict__ in, float *__restrict__ out, int A,
+                                int n, int B, int real_odd) {
+  const size_t total = (size_t)A * 2 * n * B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i % B);
+    const int f = (int)((i / B) % (2 * n));
+    const size_t a = i / ((size_t)B * 2 * n);
+    const float *c = in + a * n * B + b;
+    float v;
+    if (f == 0) {
+      v = c[0];
+    } else if (f == 2 * n - 1) {
+      v = c[(size_t)(n - 1) * B];
+    } else if (f == 1) {
+      v = up_odd(c[0], c[B]);
+    } else if (f == 2 * n - 2) {
+      v = up_even(c[(size_t)(n - 2) * B], c[(size_t)(n - 1) * B]);
+    } else {
+      const int k = f / 2;
+      if ((f & 1) == 0)
+        v = up_even(c[(size_t)(k - 1) * B], c[(size_t)k * B]);
+      else if (real_odd)
+        v = up_odd_real(c[(size_t)k * B], c[(size_t)(k + 1) * B]);
+      else
+        v = up_odd(c[(size_t)k * B], c[(size_t)(k + 1) * B]);
+    }
+    out[i] = v;
+  }
 }
 
 // x := max(x, 0) (the positivity constraint applied to a starting volume)

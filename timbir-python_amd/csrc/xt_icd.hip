@@ -139,9 +139,10 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
     return set_err(XT_E_ARG, "z_overlap_num=%d (N_t=%d, N_z=%d): only delta_z = 1 is supported",
                    S->z_overlap_num, S->N_t, O->N_z);
   if (O->N_z > 64) return set_err(XT_E_ARG, "N_z=%d > 64 slices per GPU", O->N_z);
-  if (2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA)
-    return set_err(XT_E_ARG, "delta_xy/delta_r = %g too large for one A column",
+  if (2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA_WIDE)
+    return set_err(XT_E_ARG, "delta_xy/delta_r = %g too large for one A column (at most 6)",
                    (double)(O->delta_xy / S->delta_r));
+  g->wide = 2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA;
   if (T && T->node_num > 1)
     return set_err(XT_E_ARG, "node_num=%d: this engine runs one slab on one GPU", T->node_num);
   g->N_p = S->N_p;
@@ -189,8 +190,10 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
 // dynamic LDS of k_icd_lines: per view the A column (float4, plus float2 for
 // bins 4..5 without the table) and its e offset; theta partials; neighbourhood.
 static size_t icd_lds_bytes(const XtGeom &g, int ZL, bool table) {
-  return (size_t)g.max_views * (sizeof(float4) + (table ? 0 : sizeof(float2)) + sizeof(int)) +
-         2 * 256 * sizeof(float) + 14 * ZL * sizeof(float) + 64 * sizeof(float);
+  const size_t col = g.wide ? XT_MAXA_WIDE * sizeof(float)
+                            : sizeof(float4) + (table ? 0 : sizeof(float2));
+  return (size_t)g.max_views * (col + sizeof(int)) + 2 * 256 * sizeof(float) +
+         14 * ZL * sizeof(float) + 64 * sizeof(float);
 }
 
 template <typename T>
@@ -282,8 +285,12 @@ static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const flo
                       const float *c, const float *s, const int *vf, int mode, double *out64,
                       const float *y, const float *d, float *e) {
   dim3 grid((g.N_r + XT_FP_BINS - 1) / XT_FP_BINS, g.N_p);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL>), grid, dim3(256), 0, st, g, x, Hr, c, s,
-                     vf, mode, out64, y, d, e);
+  if (g.wide)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL, XT_MAXA_WIDE>), grid, dim3(256), 0, st,
+                       g, x, Hr, c, s, vf, mode, out64, y, d, e);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL, XT_MAXA>), grid, dim3(256), 0, st, g,
+                       x, Hr, c, s, vf, mode, out64, y, d, e);
 }
 
 static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *x, const float *Hr,
@@ -311,7 +318,14 @@ static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, c
                        int gx, size_t lds) {
   const dim3 grid(gx, g.N_time), block(256);
   const bool pack = icd_packable(g, ZL, L);
-  if (B.at_vals) {
+  if (g.wide) {  // never with the table (it holds four bins per column)
+    if (pack)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true, XT_MAXA_WIDE>), grid, block,
+                         lds, st, g, B, L);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false, XT_MAXA_WIDE>), grid, block,
+                         lds, st, g, B, L);
+  } else if (B.at_vals) {
     if (pack)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, true>), grid, block, lds, st, g, B, L);
     else
@@ -710,7 +724,7 @@ static int icd_fbp(xt_icd_state *s, bool resync = true) {
 // The A-column table is used when it fits in half of the free device memory
 // (XT_OPT_ATABLE = 1, the default) and every column spans at most four bins.
 static int build_atable(xt_icd_state *s) {
-  if (g_opt[XT_OPT_ATABLE] == 0) return XT_OK;
+  if (g_opt[XT_OPT_ATABLE] == 0 || s->g.wide) return XT_OK;
   const XtGeom &g = s->g;
   const size_t n = (size_t)g.N_time * s->NYX * g.max_views;
   size_t free_b = 0, total_b = 0;
@@ -840,6 +854,15 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = upload_object(s, O, s->x))) return fail(rc);
   if ((rc = launch_detector_response(s, O->BeamWidth))) return fail(rc);
   if ((rc = build_atable(s))) return fail(rc);
+  {
+    // the line kernel stages every view's A column of a frame in LDS
+    int lds_max = 0;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->dev);
+    const size_t need = icd_lds_bytes(g, s->ZL, s->at_vals != nullptr);
+    if (lds_max > 0 && need > (size_t)lds_max)
+      return fail(set_err(XT_E_ARG, "%d views per frame need %zu B of LDS per workgroup (device "
+                          "limit %d)", g.max_views, need, lds_max));
+  }
   if ((rc = icd_resync(s))) return fail(rc);
   if (hipStreamSynchronize(s->st) != hipSuccess)
     return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
@@ -1030,6 +1053,20 @@ extern "C" int xt_icd_fbp(xt_icd_state *s) {
   return XT_OK;
 }
 
+extern "C" int xt_icd_upload(xt_icd_state *s, Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  (void)T;
+  if (!s || !S || !O) return set_err(XT_E_ARG, "null argument");
+  if (O->N_time != s->g.N_time || O->N_z != s->g.N_z || O->N_y != s->g.N_y ||
+      O->N_x != s->g.N_x || S->N_r != s->g.N_r || S->N_t != s->g.N_t)
+    return set_err(XT_E_ARG, "structs do not match the state's sizes");
+  XT_TRY(upload_object(s, O, s->x));
+  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)s->g.N_r * s->g.N_t * sizeof(float),
+                   hipMemcpyHostToDevice));
+  XT_TRY(icd_resync(s));
+  XT_HIP(hipStreamSynchronize(s->st));
+  return XT_OK;
+}
+
 extern "C" int xt_icd_unet_start(xt_icd_state *s, xt_unet *net) {
   if (!s || !net) return set_err(XT_E_ARG, "null argument");
   int d0, d1, d2;
@@ -1133,7 +1170,7 @@ static int batch_plan(xt_icd_batch *b) {
   if (g_opt[XT_OPT_BATCH_FUSE] == 0) return XT_OK;
   const xt_icd_state *s0 = b->s[0];
   for (xt_icd_state *s : b->s)
-    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd) return XT_OK;
+    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide) return XT_OK;
   for (size_t i = 0; i < b->s.size(); i++) {
     xt_icd_state *s = b->s[i];
     XtIcdLaunch L0{};
@@ -1426,11 +1463,46 @@ static int read_run_bin(const char *name, void *data, size_t count) {
   return XT_OK;
 }
 
+// initObject's upsampling of a coarse [Nzc][Nyc][Nxc] reconstruction (initICD 2:
+// y and x doubled; 3: z, y and x), F/XT_ICD_update.c:316-435,484-512.
+extern "C" int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
+                                  const Real_t *coarse, Real_t *fine) {
+  if (!coarse || !fine) return set_err(XT_E_ARG, "null argument");
+  if (init_icd != 2 && init_icd != 3) return set_err(XT_E_ARG, "initICD %d is not an upsampling", init_icd);
+  if (Nzc < (init_icd == 3 ? 2 : 1) || Nyc < 2 || Nxc < 2)
+    return set_err(XT_E_ARG, "coarse object %dx%dx%d: every upsampled size must be >= 2", Nzc,
+                   Nyc, Nxc);
+  int dev = 0;
+  XT_TRY(init_device(&dev));
+  const int Nx = 2 * Nxc, Ny = 2 * Nyc, Nz = init_icd == 3 ? 2 * Nzc : Nzc;
+  const size_t nc = (size_t)Nzc * Nyc * Nxc, nx = (size_t)Nzc * Nyc * Nx,
+               ny = (size_t)Nzc * Ny * Nx, nf = (size_t)Nz * Ny * Nx;
+  float *buf = nullptr;
+  XT_HIP(hipMalloc((void **)&buf, (nc + nx + ny + (init_icd == 3 ? nf : 0)) * sizeof(float)));
+  float *dc = buf, *dx = dc + nc, *dy = dx + nx, *dz = dy + ny;
+  hipError_t e = hipMemcpy(dc, coarse, nc * sizeof(float), hipMemcpyHostToDevice);
+  auto grid = [](size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)); };
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_upsample_axis, grid(nx), dim3(256), 0, 0, dc, dx, Nzc * Nyc, Nxc, 1, 0);
+    hipLaunchKernelGGL(k_upsample_axis, grid(ny), dim3(256), 0, 0, dx, dy, Nzc, Nyc, Nx, 1);
+    if (init_icd == 3)
+      hipLaunchKernelGGL(k_upsample_axis, grid(nf), dim3(256), 0, 0, dy, dz, 1, Nzc, Ny * Nx, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(fine, init_icd == 3 ? dz : dy, nf * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "upsampling: %s", hipGetErrorString(e));
+  return XT_OK;
+}
+
 extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   if (!S || !O || !T) return set_err(XT_E_ARG, "null argument");
   char name[256];
-  // initObject, F/XT_ICD_update.c:443-536 (initICD 0 and 1; 2 and 3 are the
-  // multi-resolution upsampling modes, not part of this path)
+  // initObject, F/XT_ICD_update.c:443-536: zero, the previous object (1), or the
+  // previous stage's coarser object upsampled in y, x (2) or z, y, x (3); the
+  // magnitude-update map is not read (every line's entry is rewritten in the
+  // first iteration before NHICD uses it)
   for (int i = 0; i < O->N_time; i++)
     memset(&O->Object[i][0][0][0], 0, (size_t)(O->N_z + 2) * O->N_y * O->N_x * sizeof(float));
   if (T->initICD == 1) {
@@ -1438,9 +1510,22 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
       snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
       XT_TRY(read_run_bin(name, &O->Object[i][1][0][0], (size_t)O->N_z * O->N_y * O->N_x));
     }
+  } else if (T->initICD == 2 || T->initICD == 3) {
+    if (O->N_y % 2 || O->N_x % 2 || (T->initICD == 3 && O->N_z % 2))
+      return set_err(XT_E_ARG, "initICD=%d needs even object sizes (%dx%dx%d)", (int)T->initICD,
+                     O->N_z, O->N_y, O->N_x);
+    const int Nzc = T->initICD == 3 ? O->N_z / 2 : O->N_z;
+    std::vector<float> coarse((size_t)Nzc * (O->N_y / 2) * (O->N_x / 2));
+    for (int i = 0; i < O->N_time; i++) {
+      snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
+      XT_TRY(read_run_bin(name, coarse.data(), coarse.size()));
+      XT_TRY(xt_upsample_object(T->initICD, Nzc, O->N_y / 2, O->N_x / 2, coarse.data(),
+                                &O->Object[i][1][0][0]));
+    }
+    dbg(T, "ICD_BackProject: initial object upsampled from the previous stage (initICD %d)\n",
+        (int)T->initICD);
   } else if (T->initICD != 0) {
-    return set_err(XT_E_ARG, "initICD=%d (multi-resolution upsampling) is not supported",
-                   (int)T->initICD);
+    return set_err(XT_E_ARG, "initICD=%d is not recognized", (int)T->initICD);
   }
   xt_icd_state *s = nullptr;
   XT_TRY(create_impl(S, O, T, &s));
@@ -1555,11 +1640,11 @@ __global__ void k_amatrix_batch(XtGeom g, const float *__restrict__ Hr, const fl
   if (i >= n) return;
   const int v = views[i];
   int first;
-  float vals[XT_MAXA];
+  float vals[XT_MAXA_WIDE];
   int cnt = xt_amatrix_column(g, Hr + (size_t)v * XT_HR_STRIDE, c[v], s[v], rows[i], cols[i], &first,
                               vals);
   int k = 0;
-  for (int b = 0; b < XT_MAXA; b++)
+  for (int b = 0; b < XT_MAXA_WIDE; b++)
     if (vals[b] > 0.f && k < max_entries) {
       index[(size_t)i * max_entries + k] = first + b;
       values[(size_t)i * max_entries + k] = vals[b];

@@ -1,0 +1,66 @@
+"""Multi-resolution ICD: the reference launcher's stage loop on the stepwise API.
+
+The launcher (4D_Recon_Launcher_for_Real_Data/XT_Initialize.py:175-198,
+XT_MBIR_Reconstruction.py:113-158) reconstructs the same sinogram in stages of
+decreasing voxel size: stage k of n uses voxels 2^(n-1-k) times the final ones
+(delta_xy scaled, N_x = N_y = N_r / delta_xy), the first from zero, every later
+one from the previous stage's object upsampled in y and x (initICD 2,
+Source_Code_4D_Fast/XT_ICD_update.c:484-512, here xt_upsample_object on the
+GPU) and the previous stage's ring offsets (updateProjOffset 3; the launcher's
+first stage estimates none, its second starts them).
+
+Here the stages pass the object and offsets in memory rather than through the
+run folder's files; `Icd.upload` places them in the next stage's state.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import synth, xt_hip
+
+
+def stage_configs(cfg: synth.ScanConfig, n_stages: int):
+    """The n stage configurations, coarsest first (the last is cfg itself)."""
+    out = []
+    for k in range(n_stages):
+        f = 2 ** (n_stages - 1 - k)
+        if cfg.N_xy % f:
+            raise ValueError(f"N_xy={cfg.N_xy} is not divisible by {f}")
+        c = dataclasses.replace(cfg, N_xy=cfg.N_xy // f, mult_xy=cfg.mult_xy * f,
+                                update_offset=0 if k == 0 else cfg.update_offset)
+        if c.N_r != cfg.N_r:
+            raise ValueError("stage sizes must keep the detector width")
+        out.append(c)
+    return out
+
+
+def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None):
+    """Runs the stages before the last one (len(iters) of them, iters[k] ICD
+    iterations each) on the run folder `path`; returns the last coarse stage's
+    object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t]."""
+    stages = stage_configs(cfg, len(iters) + 1)[:-1]
+    vol = off = None
+    for k, (c, n) in enumerate(zip(stages, iters)):
+        with xt_hip.Engine(path, c.cli_args(iterations=n)) as eng:
+            if vol is not None:
+                eng.set_volume(xt_hip.upsample_object(2, vol)[None])
+                eng.offset()[:] = off
+            with xt_hip.Icd(eng) as icd:
+                icd.iterate(n)
+                icd.download()
+            vol = eng.volume()[0].copy()
+            off = eng.offset().copy()
+        if log:
+            log(f"multi-resolution stage {k + 1}: {c.N_xy}^2 x {c.N_z}, {n} iterations")
+    return vol, off
+
+
+def start_from_coarse(icd: xt_hip.Icd, vol: np.ndarray, off: np.ndarray) -> None:
+    """The final stage's start: the coarse object upsampled (initICD 2) and the
+    coarse stage's offsets, uploaded into `icd` (e recomputed)."""
+    eng = icd.eng
+    eng.set_volume(xt_hip.upsample_object(2, vol)[None])
+    eng.offset()[:] = off
+    icd.upload()

@@ -36,6 +36,7 @@ EXPORTS = [
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
     "xt_unet_forward_host", "xt_unet_loss_grad_host", "xt_unet_train_step_host",
+    "xt_upsample_object", "xt_icd_upload",
 ]
 
 
@@ -120,6 +121,8 @@ def lib():
         L.xt_unet_train_step.argtypes = [P, P, P, C.c_float, C.POINTER(C.c_double)]
         L.xt_unet_last_ms.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.xt_icd_unet_start.argtypes = [P, P]
+        L.xt_upsample_object.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]
+        L.xt_icd_upload.argtypes = [P, P, P, P]
         L.xt_unet_forward_host.argtypes = [P, P, P]
         L.xt_unet_loss_grad_host.argtypes = [P, P, P, C.POINTER(C.c_double), P]
         L.xt_unet_train_step_host.argtypes = [P, P, P, C.c_float, C.POINTER(C.c_double)]
@@ -271,6 +274,10 @@ class Icd:
 
     def download(self) -> None:
         _check(lib().xt_icd_download(self.h, *self.eng.refs()))
+
+    def upload(self) -> None:
+        """The Engine's object and ring offsets into the state; e recomputed."""
+        _check(lib().xt_icd_upload(self.h, *self.eng.refs()))
 
     def close(self) -> None:
         if self.h:
@@ -424,6 +431,16 @@ def amatrix_columns(eng: Engine, H, rows, cols, views, max_entries=8):
                                     max_entries, cnt.ctypes.data, idx.ctypes.data,
                                     val.ctypes.data))
     return cnt, idx, val
+
+
+def upsample_object(init_icd: int, coarse) -> np.ndarray:
+    """initObject's upsampling of a coarse [Nzc][Nyc][Nxc] object: initICD 2
+    doubles y and x, 3 doubles z, y and x (F/XT_ICD_update.c:316-435)."""
+    c = np.ascontiguousarray(coarse, np.float32)
+    nz, ny, nx = c.shape
+    out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), np.float32)
+    _check(lib().xt_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data))
+    return out
 
 
 def forward_project(eng: Engine, H) -> np.ndarray:
