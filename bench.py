@@ -96,16 +96,20 @@ CPU_ICD = os.path.join(ROOT, "tests", "golden", "c2_cpu_icd_sample.npz")
 
 
 WEIGHTS = os.path.join(PKG, "weights", "unet_warmstart_256x256x16.npz")
-MULTIRES_ITERS = (20, 10)  # ICD iterations of the 128^2 and 256^2 stages before the 512^2 one
+# ICD iterations of the 128^2 and 256^2 stages before the 512^2 one: the reference
+# launcher's policy (zero start, offsets from the second stage) and the FBP-first form
+MULTIRES_ITERS = (10, 5)
+MULTIRES_FBP_ITERS = (5, 3)
 
 
 def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
     """GPU ICD one iteration at a time from a zero start (warm "zero"), from the
     filtered backprojection ("fbp"), from the warm-start network applied to
-    it ("unet") or from the reference launcher's coarser stages ("multires":
+    it ("unet"), from the reference launcher's coarser stages ("multires":
     128^2 and 256^2 reconstructions of the same sinogram, each stage upsampled
-    into the next, their set-up and iterations timed): wall time (start
-    included; the RMSE evaluations between
+    into the next, their set-up and iterations timed) or from the same stages
+    with the coarsest one started from its FBP ("multires_fbp"): wall time
+    (start included; the RMSE evaluations between
     iterations are not timed) until the volume is within tol RMSE, relative to
     the phantom's mean attenuation, of the converged MAP image (sampled voxels of
     the reference file).  Also reports the RMSE to the CPU ICD after its 30
@@ -129,9 +133,12 @@ def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
             icd.fbp()
         elif warm == "unet":
             xt_hip.icd_unet_start(icd, net)
-        elif warm == "multires":
+        elif warm in ("multires", "multires_fbp"):
             from timbir_python_amd import multires
-            vol, off = multires.coarse_stages(path, cfg, MULTIRES_ITERS)
+            fbp_first = warm == "multires_fbp"
+            vol, off = multires.coarse_stages(
+                path, cfg, MULTIRES_FBP_ITERS if fbp_first else MULTIRES_ITERS,
+                fbp_first=fbp_first)
             multires.start_from_coarse(icd, vol, off)
         icd.synchronize()
         t_init = time.perf_counter() - t0
@@ -254,10 +261,16 @@ def cpu_sample(cfg, path, target_s):
 
 
 def _ratio(conv, slow, fast):
+    """slow start's time over the fast one's; when the slow start did not reach
+    the criterion within its iteration cap, the time of its capped run gives a
+    lower bound, returned as {"at_least": r}."""
     a, b = conv.get(slow), conv.get(fast)
-    if not a or not b or not a["seconds"] or not b["seconds"]:
+    if not a or not b or not b["seconds"]:
         return None
-    return a["seconds"] / b["seconds"]
+    if a["seconds"]:
+        return a["seconds"] / b["seconds"]
+    capped = a["init_seconds"] + float(np.sum(a["times"]))
+    return {"at_least": capped / b["seconds"], "capped_iterations": len(a["times"])}
 
 
 def main():
@@ -314,7 +327,7 @@ def main():
         conv = None
         if not a.no_converge:
             conv = {}
-            for warm in ("unet", "fbp", "multires", "zero"):
+            for warm in ("multires_fbp", "unet", "fbp", "multires", "zero"):
                 c = gpu_converge(cfg, work, xt_hip, a.converge_iters, warm)
                 if c is None:
                     log(f"{warm} start not measured (no converged reference or weights)")
@@ -376,9 +389,13 @@ def main():
                       "unet: the warm-start 3D U-Net (BF16 MFMA, weights in "
                       "timbir-python_amd/weights/) on the FBP volume, inference timed; "
                       f"multires: the reference launcher's stages ({MULTIRES_ITERS[0]} ICD "
-                      f"iterations at 128^2, {MULTIRES_ITERS[1]} at 256^2, initICD-2 "
-                      "upsampling between them), stage set-up and iterations timed",
-            "speedup_unet_vs_multires": _ratio(conv, "multires", "unet"),
+                      f"iterations at 128^2 from zero without offsets, {MULTIRES_ITERS[1]} at "
+                      "256^2, initICD-2 upsampling between them), stage set-up and iterations "
+                      f"timed; multires_fbp: the same stages ({MULTIRES_FBP_ITERS[0]} and "
+                      f"{MULTIRES_FBP_ITERS[1]} iterations) with the 128^2 one started from its "
+                      "FBP and ring-offset guess",
+            "speedup_multires_fbp_vs_fbp": _ratio(conv, "fbp", "multires_fbp"),
+            "speedup_multires_fbp_vs_zero": _ratio(conv, "zero", "multires_fbp"),
             "speedup_unet_vs_fbp": _ratio(conv, "fbp", "unet"),
             "speedup_unet_vs_zero": _ratio(conv, "zero", "unet"),
             **{k: {"seconds": c["seconds"], "iterations": c["iters"],

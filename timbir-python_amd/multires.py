@@ -21,26 +21,31 @@ import numpy as np
 from . import synth, xt_hip
 
 
-def stage_configs(cfg: synth.ScanConfig, n_stages: int):
-    """The n stage configurations, coarsest first (the last is cfg itself)."""
+def stage_configs(cfg: synth.ScanConfig, n_stages: int, first_offsets: bool = False):
+    """The n stage configurations, coarsest first (the last is cfg itself); the
+    first stage estimates no ring offsets unless first_offsets (the launcher's
+    choice is not to)."""
     out = []
     for k in range(n_stages):
         f = 2 ** (n_stages - 1 - k)
         if cfg.N_xy % f:
             raise ValueError(f"N_xy={cfg.N_xy} is not divisible by {f}")
         c = dataclasses.replace(cfg, N_xy=cfg.N_xy // f, mult_xy=cfg.mult_xy * f,
-                                update_offset=0 if k == 0 else cfg.update_offset)
+                                update_offset=0 if (k == 0 and not first_offsets)
+                                else cfg.update_offset)
         if c.N_r != cfg.N_r:
             raise ValueError("stage sizes must keep the detector width")
         out.append(c)
     return out
 
 
-def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None):
+def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: bool = False):
     """Runs the stages before the last one (len(iters) of them, iters[k] ICD
     iterations each) on the run folder `path`; returns the last coarse stage's
-    object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t]."""
-    stages = stage_configs(cfg, len(iters) + 1)[:-1]
+    object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t].  fbp_first: the
+    coarsest stage starts from its filtered backprojection (with the ring-offset
+    guess, and offsets estimated from the first stage on) instead of zero."""
+    stages = stage_configs(cfg, len(iters) + 1, first_offsets=fbp_first)[:-1]
     vol = off = None
     for k, (c, n) in enumerate(zip(stages, iters)):
         with xt_hip.Engine(path, c.cli_args(iterations=n)) as eng:
@@ -48,6 +53,8 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None):
                 eng.set_volume(xt_hip.upsample_object(2, vol)[None])
                 eng.offset()[:] = off
             with xt_hip.Icd(eng) as icd:
+                if vol is None and fbp_first:
+                    icd.fbp()
                 icd.iterate(n)
                 icd.download()
             vol = eng.volume()[0].copy()
