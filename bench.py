@@ -222,6 +222,65 @@ def bench_batch(a, synth, xt_hip):
         "algorithmic_gbs": 12.0 * foot / dt / 1e9}), flush=True)
 
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def unet_flops(shape):
+    """Forward flops of the warm-start network on one volume: 2*27*C_in*C_out per
+    voxel of each 3x3x3 conv at its level, plus the 1x1x1 head."""
+    spec = [(0, 1, 16), (0, 16, 16), (1, 16, 32), (1, 32, 32), (2, 32, 64), (2, 64, 64),
+            (3, 64, 128), (3, 128, 128), (2, 192, 64), (2, 64, 64), (1, 96, 32), (1, 32, 32),
+            (0, 48, 16), (0, 16, 16)]
+    v0 = shape[0] * shape[1] * shape[2]
+    return sum(2.0 * 27 * ci * co * v0 / 8 ** lvl for lvl, ci, co in spec) + 2.0 * 16 * v0
+
+
+def bench_unet(a, synth, xt_hip):
+    """The warm-start network's kernels: inference on the 512^2x32 slab with the
+    committed weights (device time per forward from HIP events), and training
+    steps at the 256^2x16 training shape (forward + backward device time)."""
+    rng = np.random.default_rng(7)
+    inf_shape, tr_shape = (512, 512, 32), (256, 256, 16)
+    x = rng.uniform(0.0, 2e-3, inf_shape).astype(np.float32)
+    with xt_hip.UNet.load(WEIGHTS, inf_shape) as net:
+        for _ in range(a.warmup):
+            net.forward(x)
+        fwd = []
+        for _ in range(a.steps):
+            net.forward(x)
+            fwd.append(net.last_ms()[0])
+    xt = rng.uniform(0.0, 2e-3, tr_shape).astype(np.float32)
+    tt = (0.9 * xt).astype(np.float32)
+    with xt_hip.UNet(tr_shape, 1000.0, seed=1, train=True) as net:
+        for _ in range(a.warmup):
+            net.train_step(xt, tt, 1e-4)
+        tr = []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            net.train_step(xt, tt, 1e-4)
+            tr.append(net.last_ms())
+        wall = (time.perf_counter() - t0) / a.steps
+    f_ms = float(np.mean(fwd))
+    inf_tf = unet_flops(inf_shape) / (f_ms * 1e-3) / 1e12
+    tf_ms, tb_ms = float(np.mean([t[0] for t in tr])), float(np.mean([t[1] for t in tr]))
+    train_tf = 3 * unet_flops(tr_shape) / ((tf_ms + tb_ms) * 1e-3) / 1e12
+    log(f"unet: inference {f_ms:.2f} ms ({inf_tf:.0f} TFLOP/s); training step "
+        f"{tf_ms:.2f} + {tb_ms:.2f} ms ({train_tf:.0f} TFLOP/s)")
+    print(json.dumps({
+        "metric": "warm-start U-Net inference on the 512^2x32 slab (TFLOP/s)", "value": inf_tf,
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": f_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic volumes; committed weights for inference",
+        "config": {"workload": "unet3d_4lvl_16-128", "inference": list(inf_shape),
+                   "training": list(tr_shape)},
+        "roofline": {"bound": "mfma", "achieved": inf_tf, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": inf_tf / BF16_PEAK_TFLOPS, "traffic": None,
+                     "achieved_basis": "2*27*C_in*C_out flops per voxel and conv / device time "
+                                       "of the whole forward (HIP events)"},
+        "training_step": {"forward_ms": tf_ms, "backward_ms": tb_ms, "wall_ms": wall * 1e3,
+                          "tflops": train_tf}}), flush=True)
+
+
 def cpu_sample(cfg, path, target_s):
     """Oracle (C restatement, -O3, OpenMP) on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -284,10 +343,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--workdir", default=None)
-    ap.add_argument("--workload", choices=("slab", "4d", "batch"), default="slab",
+    ap.add_argument("--workload", choices=("slab", "4d", "batch", "unet"), default="slab",
                     help="slab: the 512^2x32 production slab (the metric's workload); 4d: the "
                          "256^2x16 x 16-frame interlaced reconstruction; batch: 24 uneven slabs "
-                         "advanced together (timing only for 4d and batch)")
+                         "advanced together; unet: the warm-start network's BF16 MFMA kernels "
+                         "(inference on the slab, a training step at 256^2x16) "
+                         "(timing only for 4d, batch and unet)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for the roofline traffic figure)")
@@ -310,6 +371,8 @@ def main():
     from timbir_python_amd import synth, xt_hip
     if a.workload == "batch":
         return bench_batch(a, synth, xt_hip)
+    if a.workload == "unet":
+        return bench_unet(a, synth, xt_hip)
     cfg = synth.C2 if a.workload == "slab" else synth.C3
     if a.workload != "slab":
         a.no_converge = True  # the converged-image reference exists for the slab only
