@@ -96,13 +96,17 @@ enum {
                                 homogeneous sub-iteration with the automatic number of lines
                                 in flight and with 1/2 and 1/4 of it, restores the state, and
                                 keeps the fastest (default 1) */
-  XT_OPT_BATCH_FUSE = 15     /* 1: xt_icd_batch_iterate runs each voxel-update pass of all
+  XT_OPT_BATCH_FUSE = 15,    /* 1: xt_icd_batch_iterate runs each voxel-update pass of all
                                 slabs as one launch per kernel variant (slabs of the same
                                 line width), the slabs' own small kernels between them;
                                 0: each slab's iteration as its own graph branch (default 0:
                                 one slab's lines in flight already fill the workgroup slots
                                 the line kernel's registers allow, so the fused passes run the
                                 slabs nearly in turn, with a barrier per pass) */
+  XT_OPT_LINE_THREADS = 16   /* threads of a voxel-update workgroup (one line at a time):
+                                256, 64 (one wave: 4x more lines progress at once, for
+                                short lines and few views per frame), or 0 to let the
+                                create-time autotuner choose (default 0) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);
@@ -130,6 +134,7 @@ typedef struct {
   double line_visits;           /* workgroup visits of a voxel line (a visit updates the line's
                                    slices of one z-block parity) */
   int32_t lines_in_flight;      /* voxel lines of a frame updated at once (autotuned) */
+  int32_t line_threads;         /* threads per voxel-update workgroup (autotuned) */
 } xt_icd_stats;
 
 int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state);

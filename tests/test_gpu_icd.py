@@ -53,16 +53,22 @@ def run_both(cfg, scan_root, oracle32):
     return p_gpu, p_cpu, info
 
 
-@pytest.mark.parametrize("cfg,nhicd_lines", [(dataclasses.replace(SMALL, iterations=150), 1),
-                                              (dataclasses.replace(SMALL, iterations=150), 0),
-                                              (dataclasses.replace(SMALL4D, iterations=30), 1)],
-                         ids=["small3d", "small3d-nhicd-per-block", "small4d"])
-def test_converged_parity(cfg, nhicd_lines, scan_root, oracle32, oracle64a):
+@pytest.mark.parametrize("cfg,nhicd_lines,line_threads",
+                         [(dataclasses.replace(SMALL, iterations=150), 1, 0),
+                          (dataclasses.replace(SMALL, iterations=150), 0, 0),
+                          (dataclasses.replace(SMALL, iterations=150), 1, 64),
+                          (dataclasses.replace(SMALL4D, iterations=30), 1, 0),
+                          (dataclasses.replace(SMALL4D, iterations=30), 1, 64)],
+                         ids=["small3d", "small3d-nhicd-per-block", "small3d-one-wave-lines",
+                              "small4d", "small4d-one-wave-lines"])
+def test_converged_parity(cfg, nhicd_lines, line_threads, scan_root, oracle32, oracle64a):
     xt_hip.set_option(xt_hip.OPT_NHICD_LINES, nhicd_lines)
+    xt_hip.set_option(xt_hip.OPT_LINE_THREADS, line_threads)
     try:
         p_gpu, p_cpu, info = run_both(cfg, scan_root, oracle32)
     finally:
         xt_hip.set_option(xt_hip.OPT_NHICD_LINES, 1)
+        xt_hip.set_option(xt_hip.OPT_LINE_THREADS, 0)
     xg, xc = read_object(p_gpu, cfg), read_object(p_cpu, cfg)
     ph = info["phantom"]
     mean_att = ph[ph > 0].mean()

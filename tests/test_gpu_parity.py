@@ -249,13 +249,15 @@ def test_vsc_select_bitexact(scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
-@pytest.mark.parametrize("cfg,atable", [(TINY, 1), (TINY, 0), (WIDE3, 0), (WIDE6, 0)],
-                         ids=["atable", "onthefly", "wide3", "wide6"])
+@pytest.mark.parametrize("cfg,atable,nt", [(TINY, 1, 0), (TINY, 0, 0), (WIDE3, 0, 0), (WIDE6, 0, 0),
+                                            (TINY, 1, 64), (WIDE3, 0, 64)],
+                         ids=["atable", "onthefly", "wide3", "wide6", "atable-wave", "wide3-wave"])
 @pytest.mark.parametrize("Iter", [1, 2])
-def test_update_lines_per_voxel(Iter, cfg, atable, scan_root, oracle32):
-    _print_case(f"update_lines Iter={Iter} atable={atable}", cfg)
+def test_update_lines_per_voxel(Iter, cfg, atable, nt, scan_root, oracle32):
+    _print_case(f"update_lines Iter={Iter} atable={atable} threads={nt or 256}", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     xt_hip.set_option(xt_hip.OPT_ATABLE, atable)
+    xt_hip.set_option(xt_hip.OPT_LINE_THREADS, nt)
     try:
         vol = (0.7 * info["phantom"]).astype(np.float32)
         vol[:, :, :4, :] = 0  # zero rows: zero-skipping paths
@@ -288,5 +290,6 @@ def test_update_lines_per_voxel(Iter, cfg, atable, scan_root, oracle32):
         assert abs(tot_g - tot_o) <= 1e-4 * max(1e-12, abs(tot_o))
     finally:
         xt_hip.set_option(xt_hip.OPT_ATABLE, 1)
+        xt_hip.set_option(xt_hip.OPT_LINE_THREADS, 0)
         eng.close()
         oracle32.free(S, O, T)

@@ -575,11 +575,12 @@ __device__ __forceinline__ int xt_permute(int i, int n, uint32_t key) {
 // on the host).  The body serves one slab (k_icd_lines: bx = blockIdx.x of gx
 // workgroups per frame, frame f = blockIdx.y) or one slab of a batch
 // (k_icd_lines_multi).
-template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA>
+template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA, int NT = 256>
 __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffers &B,
                                                const XtIcdLaunch &L, const int bx, const int gx,
                                                const int f) {
-  constexpr int NT = 256;
+  // NT threads per workgroup (256, or 64: one wave per line, for short lines)
+  static_assert(NT == 64 || NT == 256, "workgroup of one or four waves");
   constexpr bool WIDE = !TAB && MA > XT_MAXA;  // columns of more than 6 bins, in LDS as floats
   constexpr int ZP = PACK ? ZL / 2 : ZL;  // lanes per view group
   constexpr int G = NT / ZP;              // view groups
@@ -976,9 +977,10 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
   }
 }
 
-template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA>
-__global__ void __launch_bounds__(256) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
-  icd_lines_body<ZL, TAB, PACK, MA>(g, B, L, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
+template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA, int NT = 256>
+__global__ void __launch_bounds__(NT) k_icd_lines(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+  icd_lines_body<ZL, TAB, PACK, MA, NT>(g, B, L, (int)blockIdx.x, (int)gridDim.x,
+                                        (int)blockIdx.y);
 }
 
 // One slab of a batch launch: its state, its launch parameters, its workgroups

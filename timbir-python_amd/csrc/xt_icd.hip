@@ -62,19 +62,21 @@ int xt_report_error(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[16] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0};  // by XT_OPT_*
+static double g_opt[17] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0};  // by XT_OPT_*
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 15) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 16) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option == XT_OPT_LINE_THREADS && value != 0 && value != 64 && value != 256)
+    return set_err(XT_E_ARG, "XT_OPT_LINE_THREADS must be 0, 64 or 256");
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 15) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 16) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -189,10 +191,10 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
 
 // dynamic LDS of k_icd_lines: per view the A column (float4, plus float2 for
 // bins 4..5 without the table) and its e offset; theta partials; neighbourhood.
-static size_t icd_lds_bytes(const XtGeom &g, int ZL, bool table) {
+static size_t icd_lds_bytes(const XtGeom &g, int ZL, bool table, int nt = 256) {
   const size_t col = g.wide ? XT_MAXA_WIDE * sizeof(float)
                             : sizeof(float4) + (table ? 0 : sizeof(float2));
-  return (size_t)g.max_views * (col + sizeof(int)) + 2 * 256 * sizeof(float) +
+  return (size_t)g.max_views * (col + sizeof(int)) + 2 * (size_t)nt * sizeof(float) +
          14 * ZL * sizeof(float) + 64 * sizeof(float);
 }
 
@@ -220,6 +222,7 @@ struct xt_icd_state {
   hipStream_t st = nullptr;
   XtGeom g;
   int ZL = 32;
+  int nt = 256;  // threads per voxel-update workgroup (256 or 64)
   int K = 10, NumIter = 0, update_offset = 0, zero_mean = 0, update_var = 0, no_nhicd = 0;
   float var_est = 1.f, stop_threshold = 0.f;
   int iter = 0, stopped = 0;
@@ -313,30 +316,42 @@ static bool icd_packable(const XtGeom &g, int ZL, const XtIcdLaunch &L) {
   return g.pm != 0 && L.mode != 2;
 }
 
-template <int ZL>
-static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
-                       int gx, size_t lds) {
-  const dim3 grid(gx, g.N_time), block(256);
+template <int ZL, int NT>
+static void icd_launch_nt(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B,
+                          const XtIcdLaunch &L, int gx, size_t lds) {
+  const dim3 grid(gx, g.N_time), block(NT);
   const bool pack = icd_packable(g, ZL, L);
   if (g.wide) {  // never with the table (it holds four bins per column)
     if (pack)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true, XT_MAXA_WIDE>), grid, block,
-                         lds, st, g, B, L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true, XT_MAXA_WIDE, NT>), grid,
+                         block, lds, st, g, B, L);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false, XT_MAXA_WIDE>), grid, block,
-                         lds, st, g, B, L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false, XT_MAXA_WIDE, NT>), grid,
+                         block, lds, st, g, B, L);
   } else if (B.at_vals) {
     if (pack)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, true>), grid, block, lds, st, g, B, L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, true, XT_MAXA, NT>), grid, block,
+                         lds, st, g, B, L);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, false>), grid, block, lds, st, g, B, L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, true, false, XT_MAXA, NT>), grid, block,
+                         lds, st, g, B, L);
   } else {
     if (pack)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true>), grid, block, lds, st, g, B, L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true, XT_MAXA, NT>), grid, block,
+                         lds, st, g, B, L);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false>), grid, block, lds, st, g, B,
-                         L);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, false, XT_MAXA, NT>), grid, block,
+                         lds, st, g, B, L);
   }
+}
+
+template <int ZL>
+static void icd_launch(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B, const XtIcdLaunch &L,
+                       int gx, size_t lds, int nt) {
+  if (nt == 64)
+    icd_launch_nt<ZL, 64>(st, g, B, L, gx, lds);
+  else
+    icd_launch_nt<ZL, 256>(st, g, B, L, gx, lds);
 }
 
 static XtIcdBuffers buffers_of(xt_icd_state *s) {
@@ -367,7 +382,7 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
 // One voxel-update launch over a range of lines; gx workgroups per frame.
 static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L, int gx) {
   XtIcdBuffers B = buffers_of(s);
-  size_t lds = icd_lds_bytes(s->g, s->ZL, B.at_vals != nullptr);
+  size_t lds = icd_lds_bytes(s->g, s->ZL, B.at_vals != nullptr, s->nt);
   if (gx < 1) gx = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing) {
@@ -384,10 +399,10 @@ static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L, int gx) {
     XT_HIP(hipEventRecord(e0, s->st));
   }
   switch (s->ZL) {
-    case 8: icd_launch<8>(s->st, s->g, B, L, gx, lds); break;
-    case 16: icd_launch<16>(s->st, s->g, B, L, gx, lds); break;
-    case 32: icd_launch<32>(s->st, s->g, B, L, gx, lds); break;
-    default: icd_launch<64>(s->st, s->g, B, L, gx, lds); break;
+    case 8: icd_launch<8>(s->st, s->g, B, L, gx, lds, s->nt); break;
+    case 16: icd_launch<16>(s->st, s->g, B, L, gx, lds, s->nt); break;
+    case 32: icd_launch<32>(s->st, s->g, B, L, gx, lds, s->nt); break;
+    default: icd_launch<64>(s->st, s->g, B, L, gx, lds, s->nt); break;
   }
   XT_HIP(hipGetLastError());
   if (s->timing) XT_HIP(hipEventRecord(e1, s->st));
@@ -627,8 +642,14 @@ static void fbp_bp_launch(xt_icd_state *s, const float *q) {
 // NHICD launches, which cannot be tried before the first iteration, gain from
 // more lines in flight).
 static int icd_autotune(xt_icd_state *s) {
-  std::vector<int> cand;
-  for (int w = s->conc; w >= 16 && cand.size() < 3; w /= 2) cand.push_back(w);
+  // (lines in flight, threads per workgroup); the first is the default
+  std::vector<std::pair<int, int>> cand;
+  const int nt_opt = (int)g_opt[XT_OPT_LINE_THREADS];
+  for (int nt : {256, 64}) {
+    if (nt_opt && nt != nt_opt) continue;
+    int k = 0;
+    for (int w = s->conc; w >= 16 && k < 3; w /= 2, k++) cand.push_back({w, nt});
+  }
   if (cand.size() < 2) return XT_OK;
   const XtGeom &g = s->g;
   const size_t nmag = (size_t)g.N_time * g.nb * s->NYX;
@@ -646,11 +667,12 @@ static int icd_autotune(xt_icd_state *s) {
     rc = set_err(XT_E_HIP, "autotune snapshot");
   const int share = s->n_masked / s->K;
   float best_ms = 1e30f;
-  int best = s->conc;
+  std::pair<int, int> best = cand[0];
   const int saved_timing = s->timing;
   s->timing = 0;
   for (size_t i = 0; !rc && i < cand.size(); i++) {
-    s->conc = cand[i];
+    s->conc = cand[i].first;
+    s->nt = cand[i].second;
     XtIcdLaunch L;
     L.mode = 0;
     L.sub_iter = 0;
@@ -669,7 +691,8 @@ static int icd_autotune(xt_icd_state *s) {
       rc = set_err(XT_E_HIP, "autotune timing");
     if (!rc && !(copy(s->e, e0, s->nsino) && copy(s->x, x0, s->nobj) && copy(s->mag_blk, m0, nmag)))
       rc = set_err(XT_E_HIP, "autotune restore");
-    dbg(s->T, "autotune: %d lines in flight per frame: %.3f ms\n", cand[i], ms);
+    dbg(s->T, "autotune: %d lines in flight per frame, %d threads each: %.3f ms\n",
+        cand[i].first, cand[i].second, ms);
     if (!rc && ms < best_ms * 0.97f) {
       best_ms = ms;
       best = cand[i];
@@ -677,7 +700,8 @@ static int icd_autotune(xt_icd_state *s) {
   }
   s->timing = saved_timing;
   s->launches = 0;
-  s->conc = best;
+  s->conc = best.first;
+  s->nt = best.second;
   if (!rc && hipStreamSynchronize(s->st) != hipSuccess) rc = set_err(XT_E_HIP, "autotune sync");
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
@@ -866,6 +890,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = icd_resync(s))) return fail(rc);
   if (hipStreamSynchronize(s->st) != hipSuccess)
     return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
+  s->nt = (int)g_opt[XT_OPT_LINE_THREADS] == 64 ? 64 : 256;
   if (g_opt[XT_OPT_AUTOTUNE] != 0 && g_opt[XT_OPT_LINES] == 0 && (rc = icd_autotune(s)))
     return fail(rc);
   *out = s;
@@ -1036,6 +1061,7 @@ extern "C" int xt_icd_get_stats(xt_icd_state *s, xt_icd_stats *out) {
   out->stopped = s->stopped;
   out->line_visits = s->total_visits;
   out->lines_in_flight = s->conc;
+  out->line_threads = s->nt;
   return XT_OK;
 }
 
@@ -1170,7 +1196,7 @@ static int batch_plan(xt_icd_batch *b) {
   if (g_opt[XT_OPT_BATCH_FUSE] == 0) return XT_OK;
   const xt_icd_state *s0 = b->s[0];
   for (xt_icd_state *s : b->s)
-    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide) return XT_OK;
+    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide || s->nt != 256) return XT_OK;
   for (size_t i = 0; i < b->s.size(); i++) {
     xt_icd_state *s = b->s[i];
     XtIcdLaunch L0{};
@@ -1586,6 +1612,7 @@ static int seam_state(SeamCtx &c, Sinogram *S, ScannedObject *O, TomoInputs *T) 
   XT_TRY(make_geom(S, O, T, &c.s->g));
   XT_HIP(hipStreamCreate(&c.s->st));
   c.s->ZL = zl_for(c.s->g.N_z);
+  c.s->nt = (int)g_opt[XT_OPT_LINE_THREADS] == 64 ? 64 : 256;
   c.s->nsino = (size_t)S->N_p * S->N_r * S->N_t;
   c.s->NYX = (size_t)O->N_y * O->N_x;
   c.s->nobj = (size_t)O->N_time * c.s->NYX * O->N_z;

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
- OPT_AUTOTUNE, OPT_BATCH_FUSE) = range(1, 16)
+ OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS) = range(1, 17)
 
 # every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
@@ -49,7 +49,8 @@ class IcdStats(C.Structure):
                 ("footprint_entries", C.c_double), ("icd_kernel_ms", C.c_double),
                 ("icd_launches", C.c_int64), ("last_cost", C.c_double),
                 ("stop_percentage", C.c_double), ("stopped", C.c_int32),
-                ("line_visits", C.c_double), ("lines_in_flight", C.c_int32)]
+                ("line_visits", C.c_double), ("lines_in_flight", C.c_int32),
+                ("line_threads", C.c_int32)]
 
 
 Sinogram, ScannedObject, AMatrixCol, TomoInputs = xt_abi.abi_types(False)
