@@ -996,9 +996,9 @@ struct XtIcdSlab {
 // Every slab of a batch in one launch: grid = (max gx, sum of frames); the
 // workgroups beyond a slab's own gx return at once.  Slabs are listed by
 // increasing y0.
-template <int ZL, bool TAB, bool PACK>
-__global__ void __launch_bounds__(256) k_icd_lines_multi(const XtIcdSlab *__restrict__ slabs,
-                                                         int n) {
+template <int ZL, bool TAB, bool PACK, int NT = 256>
+__global__ void __launch_bounds__(NT) k_icd_lines_multi(const XtIcdSlab *__restrict__ slabs,
+                                                        int n) {
   int i = 0;
   while (i + 1 < n && (int)blockIdx.y >= slabs[i + 1].y0) i++;
   const int gx = slabs[i].gx;
@@ -1006,7 +1006,8 @@ __global__ void __launch_bounds__(256) k_icd_lines_multi(const XtIcdSlab *__rest
   const XtGeom g = slabs[i].g;
   const XtIcdBuffers B = slabs[i].B;
   const XtIcdLaunch L = slabs[i].L;
-  icd_lines_body<ZL, TAB, PACK>(g, B, L, (int)blockIdx.x, gx, (int)blockIdx.y - slabs[i].y0);
+  icd_lines_body<ZL, TAB, PACK, XT_MAXA, NT>(g, B, L, (int)blockIdx.x, gx,
+                                             (int)blockIdx.y - slabs[i].y0);
 }
 
 // Line selection per (frame, z-block parity): the update magnitude summed over

@@ -1173,7 +1173,7 @@ struct xt_icd_batch {
   // XtIcdSlab entries in device memory
   bool fused = false;
   struct Group {
-    int ZL = 0;
+    int ZL = 0, nt = 256;
     bool tab = false, pack = false;
     std::vector<int> slabs;
     int rows = 0;
@@ -1196,25 +1196,27 @@ static int batch_plan(xt_icd_batch *b) {
   if (g_opt[XT_OPT_BATCH_FUSE] == 0) return XT_OK;
   const xt_icd_state *s0 = b->s[0];
   for (xt_icd_state *s : b->s)
-    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide || s->nt != 256) return XT_OK;
+    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide) return XT_OK;
   for (size_t i = 0; i < b->s.size(); i++) {
     xt_icd_state *s = b->s[i];
     XtIcdLaunch L0{};
     const bool tab = s->at_vals != nullptr, pack = icd_packable(s->g, s->ZL, L0);
     size_t k = 0;
     while (k < b->groups.size() &&
-           !(b->groups[k].ZL == s->ZL && b->groups[k].tab == tab && b->groups[k].pack == pack))
+           !(b->groups[k].ZL == s->ZL && b->groups[k].tab == tab && b->groups[k].pack == pack &&
+             b->groups[k].nt == s->nt))
       k++;
     if (k == b->groups.size()) {
       b->groups.emplace_back();
       b->groups[k].ZL = s->ZL;
+      b->groups[k].nt = s->nt;
       b->groups[k].tab = tab;
       b->groups[k].pack = pack;
     }
     auto &G = b->groups[k];
     G.slabs.push_back((int)i);
     G.rows += s->g.N_time;
-    G.lds = std::max(G.lds, icd_lds_bytes(s->g, s->ZL, tab));
+    G.lds = std::max(G.lds, icd_lds_bytes(s->g, s->ZL, tab, s->nt));
   }
   const int K = s0->K, ng = (int)b->groups.size();
   std::vector<XtIcdSlab> host;
@@ -1254,26 +1256,35 @@ static int batch_plan(xt_icd_batch *b) {
   return XT_OK;
 }
 
-template <int ZL>
-static void fused_launch(hipStream_t st, const xt_icd_batch::Group &G,
-                         const xt_icd_batch::Pass &P, const XtIcdSlab *tab) {
-  const dim3 grid(P.gx_max, G.rows), block(256);
+template <int ZL, int NT>
+static void fused_launch_nt(hipStream_t st, const xt_icd_batch::Group &G,
+                            const xt_icd_batch::Pass &P, const XtIcdSlab *tab) {
+  const dim3 grid(P.gx_max, G.rows), block(NT);
   const XtIcdSlab *t = tab + P.off;
   if (G.tab) {
     if (G.pack)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, true>), grid, block, G.lds,
-                         st, t, P.n);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, true, NT>), grid, block,
+                         G.lds, st, t, P.n);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, false>), grid, block, G.lds,
-                         st, t, P.n);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, true, false, NT>), grid, block,
+                         G.lds, st, t, P.n);
   } else {
     if (G.pack)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, true>), grid, block, G.lds,
-                         st, t, P.n);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, true, NT>), grid, block,
+                         G.lds, st, t, P.n);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, false>), grid, block,
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines_multi<ZL, false, false, NT>), grid, block,
                          G.lds, st, t, P.n);
   }
+}
+
+template <int ZL>
+static void fused_launch(hipStream_t st, const xt_icd_batch::Group &G,
+                         const xt_icd_batch::Pass &P, const XtIcdSlab *tab) {
+  if (G.nt == 64)
+    fused_launch_nt<ZL, 64>(st, G, P, tab);
+  else
+    fused_launch_nt<ZL, 256>(st, G, P, tab);
 }
 
 static int fused_pass(xt_icd_batch *b, int pass) {
