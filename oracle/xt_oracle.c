@@ -990,9 +990,9 @@ static int32_t read_bin(const char *dir, const char *name, void *data, size_t n,
   return 0;
 }
 
-/* ICD driver, F/XT_ICD_update.c:928-1094.  initICD 0 (zero start) and 1 (read
- * object_n0_time_<i>.bin) are supported; 2 and 3 (multi-resolution
- * upsampling) are not part of this path. */
+/* ICD driver, F/XT_ICD_update.c:928-1094.  initICD 0 (zero start), 1 (read
+ * object_n0_time_<i>.bin), 2 and 3 (the previous stage's half-size object
+ * upsampled, oracle_upsample_object). */
 int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, const char *dir,
                                int32_t flags, double *cost_out) {
   int32_t i, j, k, l, Iter, flag = 0, iters = 0;
@@ -1026,8 +1026,21 @@ int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, con
       snprintf(name, sizeof name, "object_n%d_time_%d", T->node_rank, i);
       read_bin(dir, name, &(O->Object[i][1][0][0]), (size_t)O->N_z * O->N_y * O->N_x, T);
     }
+  } else if (T->initICD == 2 || T->initICD == 3) {
+    /* F/XT_ICD_update.c:476-512: the previous stage's half-size object, upsampled */
+    int32_t nzc = T->initICD == 3 ? O->N_z / 2 : O->N_z;
+    size_t nc = (size_t)nzc * (O->N_y / 2) * (O->N_x / 2);
+    Real_t *coarse = (Real_t *)o_calloc(nc, sizeof(Real_t));
+    for (i = 0; i < O->N_time; i++) {
+      char name[128];
+      snprintf(name, sizeof name, "object_n%d_time_%d", T->node_rank, i);
+      read_bin(dir, name, coarse, nc, T);
+      oracle_upsample_object(T->initICD, nzc, O->N_y / 2, O->N_x / 2, coarse,
+                             &(O->Object[i][1][0][0]));
+    }
+    free(coarse);
   } else if (T->initICD != 0) {
-    dbg(T, "ERROR: initObject: initICD value not supported by the oracle\n");
+    dbg(T, "ERROR: initObject: initICD value not recognized\n");
     return -1;
   }
   oracle_init_error_sino(S, O, T, H_r, E);

@@ -73,3 +73,30 @@ def test_multires_start_beats_zero_start(scan_root):
             err[start] = float(np.sqrt(np.mean((eng.volume()[0] - ref) ** 2)))
     print(f"  RMSE to the converged image after 3 iterations: {err}")
     assert err["multires"] < 0.5 * err["zero"]
+
+
+def test_two_stage_converged_parity(scan_root, oracle32, oracle64a):
+    """The launcher's two-stage run (voxels twice as wide, 40 iterations, then
+    --initICD 2 at full resolution, 120 iterations) on the GPU and in the C
+    oracle: converged objects within 1% RMSE of the phantom's mean attenuation
+    and MAP costs within 1e-4."""
+    from test_gpu_icd import map_cost, read_object
+    cfg = dataclasses.replace(SMALL, iterations=120)
+    coarse = dataclasses.replace(multires.stage_configs(cfg, 2)[0], iterations=40)
+    p_gpu, info = make_scan(cfg, scan_root, "mr-gpu")
+    p_cpu, _ = make_scan(cfg, scan_root, "mr-cpu")
+    print("[gpu] two-stage engine runs, GPU and oracle", flush=True)
+    for stage, init in ((coarse, 0), (cfg, 2)):
+        args = stage.cli_args(init_icd=init)
+        xt_hip.engine_run(p_gpu, args)
+        oracle32.run(p_cpu, args)
+    xg, xc = read_object(p_gpu, cfg), read_object(p_cpu, cfg)
+    ph = info["phantom"]
+    rmse = np.sqrt(np.mean((xg - xc) ** 2)) / ph[ph > 0].mean()
+    dg = np.fromfile(os.path.join(p_gpu, "proj_offset_n0.bin"), np.float32).reshape(cfg.N_r, cfg.N_t)
+    dc = np.fromfile(os.path.join(p_cpu, "proj_offset_n0.bin"), np.float32).reshape(cfg.N_r, cfg.N_t)
+    cg = map_cost(oracle64a, p_gpu, cfg, xg, dg)
+    cc = map_cost(oracle64a, p_cpu, cfg, xc, dc)
+    print(f"  RMSE(gpu-oracle)/mean = {rmse:.4%}, cost gpu {cg:.9g} oracle {cc:.9g}")
+    assert rmse <= 0.01
+    assert abs(cg - cc) <= 1e-4 * abs(cc)

@@ -499,3 +499,21 @@ def test_upsample_rounding_paths():
     assert (odd_real != odd_double).any()  # the distinction is real on these inputs
     even = ((b64[:, :-2] + 3.0 * b64[:, 1:-1]) / 4.0).astype(np.float32)
     np.testing.assert_array_equal(got[:, 2:-2:2], even[:, :14])
+
+
+def test_oracle_initicd2_starts_from_upsampled(scan_root):
+    """The oracle's engine with --initICD 2 and no iterations writes the
+    previous stage's half-size object upsampled by oracle_upsample_object."""
+    cfg = dataclasses.replace(TINY, iterations=2)
+    coarse = dataclasses.replace(TINY, name="tiny_coarse", N_xy=TINY.N_xy // 2,
+                                 mult_xy=2 * TINY.mult_xy, update_offset=0)
+    path, _ = make_scan(cfg, scan_root, "orc-init2")
+    o = py_oracle.Oracle(False)
+    o.run(path, coarse.cli_args(iterations=2))
+    c = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float32).reshape(
+        cfg.N_z, cfg.N_xy // 2, cfg.N_xy // 2)
+    assert np.abs(c).max() > 0
+    o.run(path, cfg.cli_args(iterations=0, init_icd=2))
+    f = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float32).reshape(
+        cfg.N_z, cfg.N_xy, cfg.N_xy)
+    np.testing.assert_array_equal(f, o.upsample_object(2, c))
