@@ -202,6 +202,21 @@ void xt_multifree(void *r, int d);
 int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
                        const Real_t *coarse, Real_t *fine);
 
+/* Real-data ingest, gen_projection_4m_HDF's arithmetic (Source_Code_4D_Fast/
+ * XT_HDFIO.c:160-199; the launcher's generate_projections,
+ * 4D_Recon_Launcher_for_Real_Data/XT_Projections.py:28-123): uint16 counts
+ * [N_p][T_in][R_in] and white / dark frames [n][T_in][R_in] of the region of
+ * interest (the frames the reference averages, already read from the HDF5
+ * file) become projection and weight [N_p][N_r][N_t] and the bright field
+ * [N_r][N_t]: per bin, ratio_r = R_in/N_r by ratio_t = T_in/N_t detector pixels
+ * summed, weight = sum |counts - dark|, bright = sum |white - dark|,
+ * projection = BH t^2 + t with t = log(bright / weight).  On the GPU; weights
+ * and bright field bit-exact with the reference's float sums. */
+int xt_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, int32_t N_t, int32_t N_r,
+                          const uint16_t *counts, int32_t n_white, const uint16_t *white,
+                          int32_t n_dark, const uint16_t *dark, double bh_quad_coef,
+                          Real_t *projection, Real_t *weight, Real_t *bright);
+
 /* DetectorResponseProfile, F/XT_Profile.c:128-217: H_r[N_p][65]. */
 int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r);
 

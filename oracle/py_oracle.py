@@ -109,6 +109,9 @@ class Oracle:
         L.oracle_parse_view_info.restype = C.c_int32
         L.oracle_bin_views.argtypes = [P, C.c_int32, C.c_int32, self.R, self.R, P, P]
         L.oracle_upsample_object.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]
+        L.oracle_counts_to_sinogram.argtypes = [C.c_int32] * 5 + [P, C.c_int32, P, C.c_int32, P,
+                                                                  C.c_double, P, P, P]
+        L.oracle_counts_to_sinogram.restype = None
         L.oracle_upsample_object.restype = None
         L.oracle_interlaced_views.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                               C.c_double, P, P, P, P, P]
@@ -126,6 +129,22 @@ class Oracle:
         out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), self.np_real)
         self.lib.oracle_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data)
         return out
+
+    def counts_to_sinogram(self, counts, white, dark, N_t, N_r, bh=0.0):
+        """gen_projection_4m_HDF's arithmetic on uint16 counts [N_p][T_in][R_in] and
+        white/dark frames [n][T_in][R_in]: (projection, weight, bright)."""
+        counts = np.ascontiguousarray(counts, np.uint16)
+        white = np.ascontiguousarray(white, np.uint16)
+        dark = np.ascontiguousarray(dark, np.uint16)
+        n_p, t_in, r_in = counts.shape
+        proj = np.zeros((n_p, N_r, N_t), self.np_real)
+        wt = np.zeros_like(proj)
+        bright = np.zeros((N_r, N_t), self.np_real)
+        self.lib.oracle_counts_to_sinogram(n_p, t_in, r_in, N_t, N_r, counts.ctypes.data,
+                                           white.shape[0], white.ctypes.data, dark.shape[0],
+                                           dark.ctypes.data, float(bh), proj.ctypes.data,
+                                           wt.ctypes.data, bright.ctypes.data)
+        return proj, wt, bright
 
     def setup(self, path, args):
         S, O, T = self.Sinogram(), self.ScannedObject(), self.TomoInputs()

@@ -1560,3 +1560,56 @@ void oracle_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t 
   }
   free(bx);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Counts to sinogram, Source_Code_4D_Fast/XT_HDFIO.c:160-199: the white and
+ * dark frames averaged per detector pixel in Real_t, then per (view, bin,
+ * row) the absolute count-minus-dark summed over the ratio_r x ratio_t pixels
+ * of the bin (weight), the same for white-minus-dark (bright), and
+ * temp = log(bright / weight) rounded to Real_t, projection =
+ * BH * temp * temp + temp in double (the macro's double literal). */
+void oracle_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, int32_t N_t, int32_t N_r,
+                               const uint16_t *counts, int32_t n_white, const uint16_t *white,
+                               int32_t n_dark, const uint16_t *dark, double bh_quad_coef,
+                               Real_t *projection, Real_t *weight, Real_t *bright) {
+  const int32_t rr = R_in / N_r, rt = T_in / N_t;
+  const size_t plane = (size_t)T_in * R_in;
+  Real_t *w2 = (Real_t *)o_calloc(plane, sizeof(Real_t));  /* [R_in][T_in] as the reference */
+  Real_t *d2 = (Real_t *)o_calloc(plane, sizeof(Real_t));
+  for (int32_t j = 0; j < R_in; j++)
+    for (int32_t k = 0; k < T_in; k++) {
+      Real_t sw = 0, sd = 0;
+      for (int32_t i = 0; i < n_white; i++) sw += white[(size_t)i * plane + (size_t)k * R_in + j];
+      for (int32_t i = 0; i < n_dark; i++) sd += dark[(size_t)i * plane + (size_t)k * R_in + j];
+      sw /= n_white;
+      sd /= n_dark;
+      w2[(size_t)j * T_in + k] = sw;
+      d2[(size_t)j * T_in + k] = sd;
+    }
+  for (int32_t j = 0; j < N_r; j++)
+    for (int32_t k = 0; k < N_t; k++) {
+      Real_t b = 0;
+      for (int32_t m = 0; m < rr; m++)
+        for (int32_t n = 0; n < rt; n++) {
+          const size_t q = (size_t)(j * rr + m) * T_in + (k * rt + n);
+          b += fabs(w2[q] - d2[q]);
+        }
+      bright[(size_t)j * N_t + k] = b;
+    }
+  for (int32_t i = 0; i < N_p; i++)
+    for (int32_t j = 0; j < N_r; j++)
+      for (int32_t k = 0; k < N_t; k++) {
+        Real_t w = 0;
+        for (int32_t m = 0; m < rr; m++)
+          for (int32_t n = 0; n < rt; n++) {
+            const int32_t r = j * rr + m, t = k * rt + n;
+            w += fabs(counts[(size_t)i * plane + (size_t)t * R_in + r] - d2[(size_t)r * T_in + t]);
+          }
+        const size_t o = ((size_t)i * N_r + j) * N_t + k;
+        Real_t temp = log(bright[(size_t)j * N_t + k] / w);
+        weight[o] = w;
+        projection[o] = bh_quad_coef * temp * temp + temp;
+      }
+  free(w2);
+  free(d2);
+}

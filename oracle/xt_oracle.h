@@ -21,6 +21,9 @@
  *   oracle_vsc_select            bit-exact vs the reference's XT_NHICD.c (oracle/_ref)
  *   oracle_parse_view_info       the reference's view_info.txt fixtures
  *   oracle_interlaced_views      golden log Source_Code_4D_Fast/info.txt:1-175
+ *   oracle_counts_to_sinogram    closed forms: flat fields give 0, Beer-Lambert ratios
+ *                                give their logarithm; block sums brute-forced
+ *                                (tests/test_oracle_pins.py)
  *   oracle_upsample_object       closed forms: constants and linear ramps reproduced
  *                                exactly (tests/test_oracle_pins.py)
  *   voxel update / surrogate     closed forms: quadratic prior + no zingers reduces ICD to
@@ -112,6 +115,17 @@ void oracle_bin_views(const Real_t *times, int32_t N_p, int32_t N_time, Real_t R
  * that is upsampled must be >= 2. */
 void oracle_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
                             const Real_t *coarse, Real_t *fine);
+
+/* Counts to sinogram, gen_projection_4m_HDF's arithmetic
+ * (Source_Code_4D_Fast/XT_HDFIO.c:160-199) on the region of interest already
+ * read: counts [N_p][T_in][R_in], white [n_white][T_in][R_in], dark
+ * [n_dark][T_in][R_in] (the frames the reference averages); outputs
+ * projection and weight [N_p][N_r][N_t] and bright [N_r][N_t], with
+ * ratio_r = R_in / N_r and ratio_t = T_in / N_t detector pixels summed per bin. */
+void oracle_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, int32_t N_t, int32_t N_r,
+                               const uint16_t *counts, int32_t n_white, const uint16_t *white,
+                               int32_t n_dark, const uint16_t *dark, double bh_quad_coef,
+                               Real_t *projection, Real_t *weight, Real_t *bright);
 
 /* Interlaced bit-reversed order and clipping (4D_Recon_Launcher_for_Real_Data/
  * XT_Interlaced_Angles.py:6-27,30-34,107-139).  Returns the kept count; the

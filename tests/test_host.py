@@ -133,3 +133,13 @@ def test_cli_binary_reports_errors(tmp_path):
     r = subprocess.run([exe, "--no_such_option", "1"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "xt_engine" in r.stderr
+
+
+def test_ingest_hdf5_needs_h5py():
+    from timbir_python_amd import ingest
+    try:
+        import h5py  # noqa: F401
+    except ImportError:
+        with pytest.raises(RuntimeError, match="h5py"):
+            ingest.read_hdf5("a.h5", "b.h5", "c.h5", N_p=1, proj_start=0, slice_begin=0,
+                             slice_num=1, x_start=0, x_width=8, N_r=8)

@@ -517,3 +517,52 @@ def test_oracle_initicd2_starts_from_upsampled(scan_root):
     f = np.fromfile(os.path.join(path, "object_n0_time_0.bin"), np.float32).reshape(
         cfg.N_z, cfg.N_xy, cfg.N_xy)
     np.testing.assert_array_equal(f, o.upsample_object(2, c))
+
+
+# ---- counts to sinogram (gen_projection_4m_HDF, F/XT_HDFIO.c:160-199) ----
+def _counts_case(seed=0, n_p=5, t_in=6, r_in=12, n_w=4, n_d=3):
+    rng = np.random.default_rng(seed)
+    white = rng.integers(3800, 4200, (n_w, t_in, r_in)).astype(np.uint16)
+    dark = rng.integers(90, 110, (n_d, t_in, r_in)).astype(np.uint16)
+    mu = rng.uniform(0.0, 2.0, (n_p, t_in, r_in))
+    counts = (dark.mean(0) + (white.mean(0) - dark.mean(0)) * np.exp(-mu)).round().astype(np.uint16)
+    return counts, white, dark
+
+
+def test_counts_to_sinogram_closed_forms():
+    o = py_oracle.Oracle(False)
+    # flat fields: counts equal to the white level, no dark -> zero projection
+    white = np.full((3, 4, 8), 2000, np.uint16)
+    dark = np.zeros((2, 4, 8), np.uint16)
+    counts = np.full((2, 4, 8), 2000, np.uint16)
+    p, w, b = o.counts_to_sinogram(counts, white, dark, N_t=2, N_r=4, bh=0.3)
+    np.testing.assert_array_equal(p, 0.0)
+    np.testing.assert_array_equal(w, 2000 * 4)  # 2 x 2 pixels per bin
+    np.testing.assert_array_equal(b, 2000 * 4)
+    # Beer-Lambert: block sums of (counts - dark) against (white - dark), BH quadratic
+    counts, white, dark = _counts_case()
+    N_t, N_r, bh = 3, 4, 0.25
+    p, w, b = o.counts_to_sinogram(counts, white, dark, N_t=N_t, N_r=N_r, bh=bh)
+    wm, dm = white.astype(np.float64).mean(0), dark.astype(np.float64).mean(0)
+    rt, rr = counts.shape[1] // N_t, counts.shape[2] // N_r
+    blk = lambda a: a.reshape(a.shape[:-2] + (N_t, rt, N_r, rr)).sum((-3, -1))
+    b_ref = blk(np.abs(wm - dm)).T
+    w_ref = np.transpose(blk(np.abs(counts - dm)), (0, 2, 1))
+    t = np.log(b_ref[None] / w_ref)
+    np.testing.assert_allclose(b, b_ref, rtol=1e-6)
+    np.testing.assert_allclose(w, w_ref, rtol=1e-6)
+    np.testing.assert_allclose(p, bh * t * t + t, rtol=2e-5, atol=2e-6)
+
+
+def test_counts_to_sinogram_pixel_mapping():
+    """One count changed at detector pixel (t, r) of view 1 moves only bin
+    (r // ratio_r, t // ratio_t) of that view."""
+    o = py_oracle.Oracle(False)
+    counts, white, dark = _counts_case(1)
+    p0, w0, _ = o.counts_to_sinogram(counts, white, dark, N_t=3, N_r=4)
+    c2 = counts.copy()
+    c2[1, 4, 7] += 50
+    p1, w1, _ = o.counts_to_sinogram(c2, white, dark, N_t=3, N_r=4)
+    changed = np.argwhere(w1 != w0)
+    assert changed.tolist() == [[1, 7 // 3, 4 // 2]]
+    assert np.isclose(w1[1, 2, 2] - w0[1, 2, 2], 50.0)

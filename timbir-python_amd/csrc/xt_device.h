@@ -1443,6 +1443,70 @@ __global__ void k_upsample_axis(const float *__restrict__ in, float *__restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Counts to sinogram, gen_projection_4m_HDF's arithmetic (F/XT_HDFIO.c:160-199),
+// on the region of interest read from the file.  Flat fields: per detector
+// pixel the mean of the white (dark) frames in float, stored [R_in][T_in] as
+// the reference; bright[j][k] and weight[i][j][k] sum |white - dark| and
+// |counts - dark| over the ratio_r x ratio_t pixels of bin (j, k) in the
+// reference's order (r offset outer, t offset inner), in float.
+__global__ void k_flat_mean(const uint16_t *__restrict__ frames, int n, int T_in, int R_in,
+                            float *__restrict__ out) {
+  const size_t plane = (size_t)T_in * R_in;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
+       q += (size_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int i = 0; i < n; i++) s += (float)frames[(size_t)i * plane + q];
+    const int t = (int)(q / R_in), r = (int)(q % R_in);
+    out[(size_t)r * T_in + t] = s / (float)n;
+  }
+}
+
+__global__ void k_bright(const float *__restrict__ w2, const float *__restrict__ d2, int T_in,
+                         int N_r, int N_t, int rr, int rt, float *__restrict__ bright) {
+  const size_t n = (size_t)N_r * N_t;
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(o / N_t), k = (int)(o % N_t);
+    float b = 0.f;
+    for (int m = 0; m < rr; m++)
+      for (int l = 0; l < rt; l++) {
+        const size_t q = (size_t)(j * rr + m) * T_in + (k * rt + l);
+        b += fabsf(w2[q] - d2[q]);
+      }
+    bright[o] = b;
+  }
+}
+
+// projection = BH * temp^2 + temp in double with temp = log(bright / weight)
+// rounded to float (the reference's Real_t temp and double macro literal)
+__global__ void k_counts_sino(const uint16_t *__restrict__ counts, const float *__restrict__ d2,
+                              const float *__restrict__ bright, int N_p, int T_in, int R_in,
+                              int N_r, int N_t, int rr, int rt, double bh,
+                              float *__restrict__ proj, float *__restrict__ weight) {
+#pragma clang fp contract(off)
+  const size_t n = (size_t)N_p * N_t * N_r;
+  const size_t plane = (size_t)T_in * R_in;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    // bins fastest: neighbouring threads read neighbouring detector pixels
+    const int j = (int)(idx % N_r);
+    const int k = (int)((idx / N_r) % N_t);
+    const int i = (int)(idx / ((size_t)N_r * N_t));
+    float w = 0.f;
+    for (int m = 0; m < rr; m++)
+      for (int l = 0; l < rt; l++) {
+        const int r = j * rr + m, t = k * rt + l;
+        w += fabsf((float)counts[(size_t)i * plane + (size_t)t * R_in + r] -
+                   d2[(size_t)r * T_in + t]);
+      }
+    const size_t o = ((size_t)i * N_r + j) * N_t + k;
+    const float temp = (float)log((double)(bright[(size_t)j * N_t + k] / w));
+    weight[o] = w;
+    proj[o] = (float)(bh * (double)temp * (double)temp + (double)temp);
+  }
+}
+
 // x := max(x, 0) (the positivity constraint applied to a starting volume)
 __global__ void k_clamp_nonneg(float *__restrict__ x, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

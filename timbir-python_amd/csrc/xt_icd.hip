@@ -1533,6 +1533,57 @@ extern "C" int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, in
   return XT_OK;
 }
 
+// gen_projection_4m_HDF's arithmetic (F/XT_HDFIO.c:160-199) on the GPU.
+extern "C" int xt_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, int32_t N_t,
+                                     int32_t N_r, const uint16_t *counts, int32_t n_white,
+                                     const uint16_t *white, int32_t n_dark, const uint16_t *dark,
+                                     double bh_quad_coef, Real_t *projection, Real_t *weight,
+                                     Real_t *bright) {
+  if (!counts || !white || !dark || !projection || !weight || !bright)
+    return set_err(XT_E_ARG, "null argument");
+  if (N_p <= 0 || N_t <= 0 || N_r <= 0 || T_in < N_t || R_in < N_r || T_in % N_t || R_in % N_r ||
+      n_white <= 0 || n_dark <= 0)
+    return set_err(XT_E_ARG, "counts %dx%dx%d do not decimate to %d bins x %d rows (or no flat "
+                   "frames: %d white, %d dark)", N_p, T_in, R_in, N_r, N_t, n_white, n_dark);
+  int dev = 0;
+  XT_TRY(init_device(&dev));
+  const size_t plane = (size_t)T_in * R_in, ns = (size_t)N_p * N_r * N_t, nb = (size_t)N_r * N_t;
+  uint16_t *dc = nullptr, *dw = nullptr, *dd = nullptr;
+  float *w2 = nullptr, *d2 = nullptr, *db = nullptr, *dp = nullptr, *dwt = nullptr;
+  hipError_t e = hipMalloc((void **)&dc, (size_t)N_p * plane * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&dw, (size_t)n_white * plane * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&dd, (size_t)n_dark * plane * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&w2, 2 * plane * sizeof(float) + nb * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&dp, 2 * ns * sizeof(float));
+  if (e == hipSuccess) {
+    d2 = w2 + plane;
+    db = d2 + plane;
+    dwt = dp + ns;
+    e = hipMemcpy(dc, counts, (size_t)N_p * plane * sizeof(uint16_t), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(dw, white, (size_t)n_white * plane * sizeof(uint16_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dd, dark, (size_t)n_dark * plane * sizeof(uint16_t), hipMemcpyHostToDevice);
+  auto grid = [](size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)); };
+  if (e == hipSuccess) {
+    const int rr = R_in / N_r, rt = T_in / N_t;
+    hipLaunchKernelGGL(k_flat_mean, grid(plane), dim3(256), 0, 0, dw, n_white, T_in, R_in, w2);
+    hipLaunchKernelGGL(k_flat_mean, grid(plane), dim3(256), 0, 0, dd, n_dark, T_in, R_in, d2);
+    hipLaunchKernelGGL(k_bright, grid(nb), dim3(256), 0, 0, w2, d2, T_in, N_r, N_t, rr, rt, db);
+    hipLaunchKernelGGL(k_counts_sino, grid(ns), dim3(256), 0, 0, dc, d2, db, N_p, T_in, R_in, N_r,
+                       N_t, rr, rt, bh_quad_coef, dp, dwt);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(projection, dp, ns * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(weight, dwt, ns * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(bright, db, nb * sizeof(float), hipMemcpyDeviceToHost);
+  for (void *p : {(void *)dc, (void *)dw, (void *)dd, (void *)w2, (void *)dp})
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "counts to sinogram: %s", hipGetErrorString(e));
+  return XT_OK;
+}
+
 extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   if (!S || !O || !T) return set_err(XT_E_ARG, "null argument");
   char name[256];

@@ -36,7 +36,7 @@ EXPORTS = [
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
     "xt_unet_forward_host", "xt_unet_loss_grad_host", "xt_unet_train_step_host",
-    "xt_upsample_object", "xt_icd_upload",
+    "xt_upsample_object", "xt_icd_upload", "xt_counts_to_sinogram",
 ]
 
 
@@ -124,6 +124,8 @@ def lib():
         L.xt_icd_unet_start.argtypes = [P, P]
         L.xt_upsample_object.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]
         L.xt_icd_upload.argtypes = [P, P, P, P]
+        L.xt_counts_to_sinogram.argtypes = [C.c_int32] * 5 + [P, C.c_int32, P, C.c_int32, P,
+                                                              C.c_double, P, P, P]
         L.xt_unet_forward_host.argtypes = [P, P, P]
         L.xt_unet_loss_grad_host.argtypes = [P, P, P, C.POINTER(C.c_double), P]
         L.xt_unet_train_step_host.argtypes = [P, P, P, C.c_float, C.POINTER(C.c_double)]
@@ -432,6 +434,24 @@ def amatrix_columns(eng: Engine, H, rows, cols, views, max_entries=8):
                                     max_entries, cnt.ctypes.data, idx.ctypes.data,
                                     val.ctypes.data))
     return cnt, idx, val
+
+
+def counts_to_sinogram(counts, white, dark, N_t: int, N_r: int, bh: float = 0.0):
+    """Real-data ingest on the GPU (F/XT_HDFIO.c:160-199): uint16 counts
+    [N_p][T_in][R_in] and white/dark frames [n][T_in][R_in] to (projection,
+    weight [N_p][N_r][N_t], bright [N_r][N_t]) float32."""
+    counts = np.ascontiguousarray(counts, np.uint16)
+    white = np.ascontiguousarray(white, np.uint16)
+    dark = np.ascontiguousarray(dark, np.uint16)
+    n_p, t_in, r_in = counts.shape
+    proj = np.zeros((n_p, N_r, N_t), np.float32)
+    wt = np.zeros_like(proj)
+    bright = np.zeros((N_r, N_t), np.float32)
+    _check(lib().xt_counts_to_sinogram(n_p, t_in, r_in, N_t, N_r, counts.ctypes.data,
+                                       white.shape[0], white.ctypes.data, dark.shape[0],
+                                       dark.ctypes.data, float(bh), proj.ctypes.data,
+                                       wt.ctypes.data, bright.ctypes.data))
+    return proj, wt, bright
 
 
 def upsample_object(init_icd: int, coarse) -> np.ndarray:
