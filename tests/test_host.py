@@ -143,3 +143,24 @@ def test_ingest_hdf5_needs_h5py():
         with pytest.raises(RuntimeError, match="h5py"):
             ingest.read_hdf5("a.h5", "b.h5", "c.h5", N_p=1, proj_start=0, slice_begin=0,
                              slice_num=1, x_start=0, x_width=8, N_r=8)
+
+
+def test_output_writers(tmp_path):
+    """The launcher's result formats: air -> 0 HU and water -> 1000 HU, the
+    8-bit window clipped at both ends, TIFF slices that read back."""
+    from PIL import Image
+    from timbir_python_amd import output
+    att, den = 100.0 ** 4, 0.001 ** 4
+    air = 0.496372335005353 * att * 0.001205 * den
+    water = 0.521225397034623 * att * 1.0 * den
+    np.testing.assert_allclose(output.um_to_hu([air, water]), [0.0, 1000.0], atol=1e-9)
+    np.testing.assert_array_equal(output.hu_to_uint8(np.array([-50.0, 0.0, 500.0, 1000.0, 2e3]),
+                                                     1000.0, 0.0), [0, 0, 127, 255, 255])
+    obj = np.zeros((2, 3, 4, 4), np.float32)
+    obj[1, 2, 1, 1] = water
+    for j in range(2):
+        obj[j].tofile(tmp_path / f"object_n0_time_{j}.bin")
+    names = output.write_tiff_slices(str(tmp_path), str(tmp_path / "tif"), 2, 3, 4, 1000.0, -1000.0)
+    assert len(names) == 6
+    img = np.asarray(Image.open(tmp_path / "tif" / "object_time_z_1_2.tif"))
+    assert img.dtype == np.uint8 and img[1, 1] == 255 and img[0, 0] == 127
