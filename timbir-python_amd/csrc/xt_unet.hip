@@ -160,6 +160,108 @@ __global__ void __launch_bounds__(256) k_conv3(const bf16_t *__restrict__ in, in
   }
 }
 
+// LDS-tiled variant for Ci % 16 == 0: a workgroup's 64*MB voxels form a TY x TX
+// tile of one d0-plane (TX along d2); per chunk of CC input channels the tile
+// and its one-voxel halo in d0, d1 and d2 are staged in LDS once and every
+// tap's A fragment is read from there (the direct kernel reads each input 27
+// times from L2).  K order: channel chunks outer, taps inner (CC = 32: one tap
+// per MFMA K-step; CC = 16: two).  TY | d1 and TX | d2.
+template <int MB, int NB, int CC>
+__global__ void __launch_bounds__(256) k_conv3_lds(const bf16_t *__restrict__ in, int Ci,
+                                                   const bf16_t *__restrict__ W, int Kpad,
+                                                   const float *__restrict__ bias, int Co, int d0,
+                                                   int d1, int d2, int TY, int TX,
+                                                   bf16_t *__restrict__ out, int relu,
+                                                   const bf16_t *__restrict__ mask) {
+  constexpr int CP = CC + 8;  // padded channel stride of a staged voxel (bf16)
+  constexpr int TPS = 32 / CC;
+  extern __shared__ uint4 lds4[];
+  bf16_t *tile = (bf16_t *)lds4;  // [3][TY+2][TX+2][CP]
+  const int P = d1 * d2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int n0 = blockIdx.y * (16 * NB);
+  const int ntx = d2 / TX, nty = d1 / TY;
+  int bi = blockIdx.x;
+  const int bx = bi % ntx;
+  bi /= ntx;
+  const int by = bi % nty;
+  const int p0 = bi / nty;
+  const int y0 = by * TY, x0 = bx * TX;
+  const int HY = TY + 2, HX = TX + 2, HV = 3 * HY * HX;
+  int ty[MB], tx[MB];
+  bool rok[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++) {
+    const int m = (wave * MB + mb) * 16 + r;
+    ty[mb] = m / TX;
+    tx[mb] = m % TX;
+    rok[mb] = ty[mb] < TY;
+  }
+  f32x4 acc[MB][NB];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  for (int c0 = 0; c0 < Ci; c0 += CC) {
+    __syncthreads();  // the previous chunk's fragments are read
+    for (int q = threadIdx.x; q < HV * (CC / 8); q += 256) {
+      const int hv = q / (CC / 8), c8 = q % (CC / 8);
+      const int hz = hv / (HY * HX), rem = hv % (HY * HX);
+      const int pp = p0 + hz - 1, yy = y0 + rem / HX - 1, xx = x0 + rem % HX - 1;
+      uint4 v = zero4;
+      if ((unsigned)pp < (unsigned)d0 && (unsigned)yy < (unsigned)d1 && (unsigned)xx < (unsigned)d2)
+        v = *(const uint4 *)(in + ((size_t)pp * P + (size_t)yy * d2 + xx) * Ci + c0 + 8 * c8);
+      *(uint4 *)(tile + (size_t)hv * CP + 8 * c8) = v;
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < 27; t0 += TPS) {
+      const int tap = t0 + (TPS == 2 ? (g >> 1) : 0);
+      const int cg = (TPS == 2 ? (g & 1) : g) * 8;
+      const bool tok = tap < 27;
+      const int a0 = tap / 9, a1 = (tap / 3) % 3, a2 = tap % 3;  // halo offsets 0..2
+      bf16x8 a[MB], b[NB];
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+        const int hv = (a0 * HY + ty[mb] + a1) * HX + tx[mb] + a2;
+        a[mb] = as_bf16x8((tok && rok[mb]) ? *(const uint4 *)(tile + (size_t)hv * CP + cg) : zero4);
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const int co = n0 + nb * 16 + r;
+        b[nb] = as_bf16x8((tok && co < Co)
+                              ? *(const uint4 *)(W + (size_t)co * Kpad + (size_t)tap * Ci + c0 + cg)
+                              : zero4);
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++) {
+    const int co = n0 + nb * 16 + r;
+    if (co >= Co) continue;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int m = (wave * MB + mb) * 16 + g * 4 + i;
+        const int yy = m / TX, xx = m % TX;
+        if (yy >= TY) continue;
+        const size_t v = ((size_t)p0 * d1 + y0 + yy) * d2 + x0 + xx;
+        float val = acc[mb][nb][i] + bv;
+        if (relu) val = fmaxf(val, 0.f);
+        if (mask && !bf_pos(mask[v * Co + co])) val = 0.f;
+        out[v * Co + co] = f2bf(val);
+      }
+  }
+}
+
 // Weight gradient: M = output channels (16*MBo per wave), N = kk columns (16*NBk
 // per wave), K = voxels in steps of 32.  Each workgroup reduces one chunk of
 // voxels (split over its 4 waves) and adds its partial tile to dW with FP32
@@ -281,43 +383,58 @@ __global__ void __launch_bounds__(256) k_bias_grad(const bf16_t *__restrict__ go
   }
 }
 
-// 2x2x2 average pool: fine dims (d0, d1, d2) -> coarse (d0/2, d1/2, d2/2)
+// 2x2x2 average pool: fine dims (d0, d1, d2) -> coarse (d0/2, d1/2, d2/2); one
+// thread per coarse voxel and 8 channels (C % 8 == 0), 16-byte loads and stores
 __global__ void k_pool(const bf16_t *__restrict__ in, int C, int d0, int d1, int d2,
                        bf16_t *__restrict__ out) {
-  const int e0 = d0 / 2, e1 = d1 / 2, e2 = d2 / 2;
-  const size_t n = (size_t)e0 * e1 * e2 * C;
+  const int e1 = d1 / 2, e2 = d2 / 2, C8 = C / 8;
+  const size_t n = (size_t)(d0 / 2) * e1 * e2 * C8;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const size_t w = i / C;
-    const int q2 = (int)(w % e2), q1 = (int)((w / e2) % e1), q0 = (int)(w / ((size_t)e1 * e2));
-    float s = 0.f;
+    const int c8 = (int)(i % C8);
+    const int w = (int)(i / C8);  // coarse voxel (V < 2^31)
+    const int q2 = w % e2, q1 = (w / e2) % e1, q0 = w / (e1 * e2);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const size_t v = ((size_t)(2 * q0 + (k >> 2)) * d1 + (2 * q1 + ((k >> 1) & 1))) * d2 +
                        (2 * q2 + (k & 1));
-      s += bf2f(in[v * C + c]);
+      const uint4 q = *(const uint4 *)(in + v * C + 8 * c8);
+      const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        s[2 * j] += bf2f((bf16_t)(u[j] & 0xFFFFu));
+        s[2 * j + 1] += bf2f((bf16_t)(u[j] >> 16));
+      }
     }
-    out[i] = f2bf(s * 0.125f);
+    uint4 o;
+    uint32_t *ou = (uint32_t *)&o;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      ou[j] = (uint32_t)f2bf(s[2 * j] * 0.125f) | ((uint32_t)f2bf(s[2 * j + 1] * 0.125f) << 16);
+    *(uint4 *)(out + (size_t)w * C + 8 * c8) = o;
   }
 }
 
-// out[v] = [ low[parent(v)] (Ca channels) | skip[v] (Cb channels) ] on fine dims
+// out[v] = [ low[parent(v)] (Ca channels) | skip[v] (Cb channels) ] on fine dims;
+// one thread per 8 channels (Ca, Cb % 8 == 0), 16-byte copies
 __global__ void k_upcat(const bf16_t *__restrict__ low, int Ca, const bf16_t *__restrict__ skip,
                         int Cb, int d0, int d1, int d2, bf16_t *__restrict__ out) {
-  const int Ct = Ca + Cb;
-  const size_t n = (size_t)d0 * d1 * d2 * Ct;
+  const int Ct8 = (Ca + Cb) / 8, Ca8 = Ca / 8;
+  const size_t n = (size_t)d0 * d1 * d2 * Ct8;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Ct);
-    const size_t v = i / Ct;
-    if (c < Ca) {
-      const int p2 = (int)(v % d2), p1 = (int)((v / d2) % d1), p0 = (int)(v / ((size_t)d1 * d2));
+    const int c8 = (int)(i % Ct8);
+    const int v = (int)(i / Ct8);  // V < 2^31
+    uint4 q;
+    if (c8 < Ca8) {
+      const int p2 = v % d2, p1 = (v / d2) % d1, p0 = v / (d1 * d2);
       const size_t u = ((size_t)(p0 / 2) * (d1 / 2) + p1 / 2) * (d2 / 2) + p2 / 2;
-      out[i] = low[u * Ca + c];
+      q = *(const uint4 *)(low + u * Ca + 8 * c8);
     } else {
-      out[i] = skip[v * Cb + (c - Ca)];
+      q = *(const uint4 *)(skip + (size_t)v * Cb + 8 * (c8 - Ca8));
     }
+    *(uint4 *)(out + i * 8) = q;
   }
 }
 
@@ -371,10 +488,17 @@ __global__ void k_prep(const float *__restrict__ x, float scale, size_t n, bf16_
 __global__ void k_head_fwd(const bf16_t *__restrict__ d, int C, const float *__restrict__ wh,
                            const float *__restrict__ bh, const float *x, float scale, size_t n,
                            float *out) {
+  // C == 16: two 16-byte loads per voxel
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (size_t)gridDim.x * blockDim.x) {
+    const uint4 q0 = *(const uint4 *)(d + v * C), q1 = *(const uint4 *)(d + v * C + 8);
+    const uint32_t u[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
     float h = bh[0];
-    for (int c = 0; c < C; c++) h += bf2f(d[v * C + c]) * wh[c];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      h += bf2f((bf16_t)(u[j] & 0xFFFFu)) * wh[2 * j];
+      h += bf2f((bf16_t)(u[j] >> 16)) * wh[2 * j + 1];
+    }
     out[v] = (x[v] * scale + h) / scale;
   }
 }
@@ -523,6 +647,7 @@ struct xt_unet {
   float *stage = nullptr;  // [2][V0] host-call staging (input, target/output)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double fwd_ms = 0, bwd_ms = 0;
+  bool lds_conv = true;  // LDS-tiled convolutions where Ci % 16 == 0 and Ci >= 32
 };
 
 namespace {
@@ -540,11 +665,38 @@ struct Carver {
   }
 };
 
+int pow2_divisor(int n, int cap) {  // largest power of two dividing n, at most cap
+  int p = 1;
+  while (p * 2 <= cap && n % (p * 2) == 0) p *= 2;
+  return p;
+}
+
 void launch_conv(xt_unet *u, hipStream_t st, const bf16_t *in, int Ci, const bf16_t *W, int Kpad,
                  const float *bias, int Co, const int *d, bf16_t *out, int relu,
                  const bf16_t *mask) {
   const int V = d[0] * d[1] * d[2];
   const int NB = (Co % 64 == 0) ? 4 : (Co % 32 == 0) ? 2 : 1;
+  // LDS tiles pay from 32 input channels on: with 16 the 4x halo staging costs
+  // more than the L2 re-reads it saves (16->16 at 512x512x32: 785 against 547 us)
+  if (Ci % 16 == 0 && Ci >= 32 && u->lds_conv) {
+    const int CC = (Ci % 32 == 0) ? 32 : 16;
+    const int MBl = (NB == 1 && CC == 16) ? 4 : 2;
+    const int TX = pow2_divisor(d[2], std::min(32, 64 * MBl));
+    const int TY = pow2_divisor(d[1], 64 * MBl / TX);
+    const size_t lds = (size_t)3 * (TY + 2) * (TX + 2) * (CC + 8) * sizeof(bf16_t);
+    dim3 grid(d[0] * (d[1] / TY) * (d[2] / TX), Co / (16 * NB));
+#define UN_LDS(MB_, NB_, CC_)                                                                     \
+  hipLaunchKernelGGL((k_conv3_lds<MB_, NB_, CC_>), grid, dim3(256), lds, st, in, Ci, W, Kpad, bias, \
+                     Co, d[0], d[1], d[2], TY, TX, out, relu, mask)
+    if (NB == 1 && CC == 16) UN_LDS(4, 1, 16);
+    else if (NB == 1) UN_LDS(2, 1, 32);
+    else if (NB == 2 && CC == 16) UN_LDS(2, 2, 16);
+    else if (NB == 2) UN_LDS(2, 2, 32);
+    else if (CC == 16) UN_LDS(2, 4, 16);
+    else UN_LDS(2, 4, 32);
+#undef UN_LDS
+    return;
+  }
   const int MB = 8 / NB;
   dim3 grid((V + 64 * MB - 1) / (64 * MB), Co / (16 * NB));
   if (Ci % 8 != 0) {
@@ -609,12 +761,12 @@ void enqueue_forward(xt_unet *u, const float *x, float *out, hipStream_t st) {
   };
   auto pool = [&](int lvl, const bf16_t *in, int C, bf16_t *o) {
     const int *d = u->dims[lvl];
-    hipLaunchKernelGGL(k_pool, dim3(grid1(u->V[lvl + 1] * C)), dim3(256), 0, st, in, C, d[0], d[1],
-                       d[2], o);
+    hipLaunchKernelGGL(k_pool, dim3(grid1(u->V[lvl + 1] * C / 8)), dim3(256), 0, st, in, C, d[0],
+                       d[1], d[2], o);
   };
   auto upcat = [&](int lvl, const bf16_t *low, int Ca, const bf16_t *skip, int Cb, bf16_t *o) {
     const int *d = u->dims[lvl];
-    hipLaunchKernelGGL(k_upcat, dim3(grid1(u->V[lvl] * (Ca + Cb))), dim3(256), 0, st, low, Ca,
+    hipLaunchKernelGGL(k_upcat, dim3(grid1(u->V[lvl] * (Ca + Cb) / 8)), dim3(256), 0, st, low, Ca,
                        skip, Cb, d[0], d[1], d[2], o);
   };
   conv(0, u->t_in, u->e1a);
