@@ -103,10 +103,15 @@ enum {
                                 one slab's lines in flight already fill the workgroup slots
                                 the line kernel's registers allow, so the fused passes run the
                                 slabs nearly in turn, with a barrier per pass) */
-  XT_OPT_LINE_THREADS = 16   /* threads of a voxel-update workgroup (one line at a time):
+  XT_OPT_LINE_THREADS = 16,  /* threads of a voxel-update workgroup (one line at a time):
                                 256, 64 (one wave: 4x more lines progress at once, for
                                 short lines and few views per frame), or 0 to let the
                                 create-time autotuner choose (default 0) */
+  XT_OPT_PHASE_CONTRAST = 17 /* 1: phase-contrast tomography (the reference's
+                                -DPHASE_CONTRAST_TOMOGRAPHY build): every footprint is the
+                                detector Laplacian of the absorption one (Lap_Kernel), slices
+                                of a line are updated in turn; absorption starts (filtered
+                                backprojection, network) are refused (default 0) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);

@@ -112,6 +112,8 @@ class Oracle:
         L.oracle_counts_to_sinogram.argtypes = [C.c_int32] * 5 + [P, C.c_int32, P, C.c_int32, P,
                                                                   C.c_double, P, P, P]
         L.oracle_counts_to_sinogram.restype = None
+        L.oracle_set_phase_contrast.argtypes = [C.c_int32]
+        L.oracle_set_phase_contrast.restype = None
         L.oracle_upsample_object.restype = None
         L.oracle_interlaced_views.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                               C.c_double, P, P, P, P, P]
@@ -129,6 +131,10 @@ class Oracle:
         out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), self.np_real)
         self.lib.oracle_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data)
         return out
+
+    def set_phase_contrast(self, on):
+        """The -DPHASE_CONTRAST_TOMOGRAPHY build's Laplacian footprints on/off."""
+        self.lib.oracle_set_phase_contrast(1 if on else 0)
 
     def counts_to_sinogram(self, counts, white, dark, N_t, N_r, bh=0.0):
         """gen_projection_4m_HDF's arithmetic on uint16 counts [N_p][T_in][R_in] and

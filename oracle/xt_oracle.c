@@ -250,6 +250,62 @@ void oracle_amatrix_column(Sinogram *S, ScannedObject *O, Real_t **H_r, AMatrixC
   Ai->count = (uint8_t)count;
 }
 
+/* Phase-contrast tomography (the reference's -DPHASE_CONTRAST_TOMOGRAPHY build):
+ * every voxel's system-matrix footprint is the detector Laplacian of its
+ * absorption footprint. */
+static int32_t g_phase_contrast = 0;
+void oracle_set_phase_contrast(int32_t on) { g_phase_contrast = on != 0; }
+
+#define PC_MAXR 24
+#define PC_MAXT 4
+/* compute_2DAMatrixLine + compute_LapMatrix_4m_AMatrix, F/XT_AMatrix.c:113-187:
+ * the column's entries (times delta_t) as a line over bins index[0].., then
+ * L[i][j] = sum over m, n of Lap_Kernel[m+1][n+1] * line[i + m - 1] for the
+ * line's rows (col j + n - 1 inside the voxel's z_overlap rows), on the grid
+ * grown by one bin and one row on each side, trimmed to the detector.  Lm is
+ * [r_num][t_num] for bins r_start.. and rows t_start.. */
+static void pc_footprint(Sinogram *S, const AMatrixCol *A, int32_t slice, int32_t *r_start,
+                         int32_t *r_num, int32_t *t_start, int32_t *t_num,
+                         Real_t Lm[PC_MAXR][PC_MAXT]) {
+  int32_t i, j, m, n, row, col;
+  Real_t line[PC_MAXR];
+  *t_start = slice * S->z_overlap_num;
+  *t_num = S->z_overlap_num;
+  *r_num = A->count;
+  *r_start = A->count ? A->index[0] : 0;
+  if (*r_num == 0 || *t_num == 0) return;
+  memset(line, 0, sizeof line);
+  for (i = 0; i < A->count; i++) line[A->index[i] - *r_start] = A->values[i] * S->delta_t;
+  memset(Lm, 0, sizeof(Real_t) * PC_MAXR * PC_MAXT);
+  for (i = 0; i < *r_num + 2; i++)
+    for (j = 0; j < *t_num + 2; j++)
+      for (m = -1; m <= 1; m++)
+        for (n = -1; n <= 1; n++) {
+          row = i + m - 1;
+          col = j + n - 1;
+          if (row >= 0 && col >= 0 && row < *r_num && col < *t_num)
+            Lm[i][j] += S->Lap_Kernel[m + 1][n + 1] * line[row];
+        }
+  *r_num += 2;
+  *t_num += 2;
+  *r_start -= 1;
+  *t_start -= 1;
+  if (*r_start < 0) {
+    for (i = 0; i < *r_num - 1; i++)
+      for (j = 0; j < *t_num; j++) Lm[i][j] = Lm[i + 1][j];
+    *r_start = 0;
+    *r_num -= 1;
+  }
+  if (*t_start < 0) {
+    for (i = 0; i < *r_num; i++)
+      for (j = 0; j < *t_num - 1; j++) Lm[i][j] = Lm[i][j + 1];
+    *t_start = 0;
+    *t_num -= 1;
+  }
+  if (*r_start + *r_num > S->N_r) *r_num -= 1;
+  if (*t_start + *t_num > S->N_t) *t_num -= 1;
+}
+
 static int32_t max_entries_per_column(Sinogram *S, ScannedObject *O) {
   /* AvgNumXElements, F/XT_VoxUpdate.c:109 */
   return (uint8_t)ceil(3 * O->delta_xy / S->delta_r);
@@ -275,6 +331,15 @@ void oracle_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, Real_t 
           oracle_amatrix_column(S, O, H_r, &A, j, k, sino_idx);
           for (int32_t slice = 0; slice < O->N_z; slice++) {
             Real_t pixel = O->Object[i][slice + 1][j][k];
+            if (g_phase_contrast) {
+              /* forward_project_voxel_AMat2D, F/XT_ForwardProject.c:27-44 */
+              Real_t Lm[PC_MAXR][PC_MAXT];
+              int32_t rs, rn, ts, tn;
+              pc_footprint(S, &A, slice, &rs, &rn, &ts, &tn, Lm);
+              for (int32_t m = 0; m < rn; m++)
+                for (int32_t n = 0; n < tn; n++) AX[sino_idx][rs + m][ts + n] += pixel * Lm[m][n];
+              continue;
+            }
             for (int32_t m = 0; m < A.count; m++) {
               int32_t idx = A.index[m];
               Real_t val = A.values[m];
@@ -400,6 +465,51 @@ static void compute_voxel_update(Sinogram *S, ScannedObject *O, TomoInputs *T, R
   z_overlap_num = S->z_overlap_num;
   THETA1 = 0.0;
   THETA2 = 0.0;
+  if (g_phase_contrast) {
+    /* compute_voxel_update_AMat2D, F/XT_VoxUpdate.c:590-690 */
+    Real_t Lm[PC_MAXR][PC_MAXT];
+    int32_t rs, rnum, ts, tnum;
+    for (p = 0; p < O->ProjNum[i_new]; p++) {
+      sino_view = O->ProjIdxPtr[i_new][p];
+      pc_footprint(S, &A[p], slice, &rs, &rnum, &ts, &tnum, Lm);
+      for (q = 0; q < rnum; q++) {
+        i_r = rs + q;
+        for (r = 0; r < tnum; r++) {
+          ProjectionEntry = Lm[q][r];
+          i_t = ts + r;
+          if (S->ProjSelect[sino_view][i_r][i_t] == true) {
+            THETA2 += (ProjectionEntry * ProjectionEntry * T->Weight[sino_view][i_r][i_t]);
+            THETA1 += -(E[sino_view][i_r][i_t] * ProjectionEntry * T->Weight[sino_view][i_r][i_t]);
+          } else {
+            THETASelTemp = T->ErrorSinoThresh * T->ErrorSinoDelta *
+                           sqrt(T->Weight[sino_view][i_r][i_t]) / fabs(E[sino_view][i_r][i_t]);
+            THETA2 += (ProjectionEntry * ProjectionEntry * THETASelTemp);
+            THETA1 += -(E[sino_view][i_r][i_t] * ProjectionEntry * THETASelTemp);
+          }
+        }
+      }
+    }
+    UpdatedVoxelValue =
+        oracle_functional_substitution(V, THETA1, THETA2, O, T, nhood27, flag27, tn, tf, positivity);
+    O->Object[i_new][slice + 1][j_new][k_new] = UpdatedVoxelValue;
+    for (p = 0; p < O->ProjNum[i_new]; p++) {
+      sino_view = O->ProjIdxPtr[i_new][p];
+      pc_footprint(S, &A[p], slice, &rs, &rnum, &ts, &tnum, Lm);
+      for (q = 0; q < rnum; q++) {
+        i_r = rs + q;
+        for (r = 0; r < tnum; r++) {
+          i_t = ts + r;
+          E[sino_view][i_r][i_t] -= (Lm[q][r] * (O->Object[i_new][slice + 1][j_new][k_new] - V));
+          if (fabs(E[sino_view][i_r][i_t] * sqrt(T->Weight[sino_view][i_r][i_t])) <
+              T->ErrorSinoThresh)
+            S->ProjSelect[sino_view][i_r][i_t] = true;
+          else
+            S->ProjSelect[sino_view][i_r][i_t] = false;
+        }
+      }
+    }
+    return;
+  }
   for (p = 0; p < O->ProjNum[i_new]; p++) {
     sino_view = O->ProjIdxPtr[i_new][p];
     for (q = 0; q < A[p].count; q++) {
@@ -1352,6 +1462,12 @@ static int32_t init_structures(const char *dir, Sinogram *S, ScannedObject *O, T
   if (S->N_r % (int32_t)O->delta_xy != 0) return -1;
   S->delta_r = S->Length_R / (S->N_r);
   S->delta_t = S->Length_T / (S->N_t);
+  {
+    /* F/XT_Init.c:284,439-441 */
+    Real_t lap[3][3] = {{-0.5, -1, -0.5}, {-1, 6, -1}, {-0.5, -1, -0.5}};
+    for (int32_t i = 0; i < 3; i++)
+      for (int32_t j = 0; j < 3; j++) S->Lap_Kernel[i][j] = lap[i][j] / (S->delta_r * S->delta_r);
+  }
   S->R0 = -T->RotCenter * S->delta_r;
   S->RMax = (S->N_r - T->RotCenter) * S->delta_r;
   S->T0 = -S->Length_T / 2.0;

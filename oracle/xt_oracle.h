@@ -116,6 +116,11 @@ void oracle_bin_views(const Real_t *times, int32_t N_p, int32_t N_time, Real_t R
 void oracle_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, int32_t Nxc,
                             const Real_t *coarse, Real_t *fine);
 
+/* Phase-contrast tomography on (1) or off (0) for every later call (the
+ * reference's -DPHASE_CONTRAST_TOMOGRAPHY build: Laplacian footprints,
+ * F/XT_AMatrix.c:113-187, F/XT_ForwardProject.c:27-44, F/XT_VoxUpdate.c:590-845). */
+void oracle_set_phase_contrast(int32_t on);
+
 /* Counts to sinogram, gen_projection_4m_HDF's arithmetic
  * (Source_Code_4D_Fast/XT_HDFIO.c:160-199) on the region of interest already
  * read: counts [N_p][T_in][R_in], white [n_white][T_in][R_in], dark

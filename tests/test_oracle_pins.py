@@ -566,3 +566,50 @@ def test_counts_to_sinogram_pixel_mapping():
     changed = np.argwhere(w1 != w0)
     assert changed.tolist() == [[1, 7 // 3, 4 // 2]]
     assert np.isclose(w1[1, 2, 2] - w0[1, 2, 2], 50.0)
+
+
+# ---- phase contrast (F/XT_AMatrix.c:113-187, F/XT_ForwardProject.c:27-44) ----
+def _lap_filter(ax, dr):
+    """The detector Laplacian (F/XT_Init.c:284) of every view, zero outside."""
+    k = np.array([[-0.5, -1, -0.5], [-1, 6, -1], [-0.5, -1, -0.5]]) / (dr * dr)
+    out = np.zeros_like(ax)
+    n_r, n_t = ax.shape[1:]
+    for a in range(3):
+        for b in range(3):
+            src = np.zeros_like(ax)
+            r0, r1 = max(0, a - 1), min(n_r, n_r + a - 1)
+            t0, t1 = max(0, b - 1), min(n_t, n_t + b - 1)
+            src[:, r0 - (a - 1):r1 - (a - 1), t0 - (b - 1):t1 - (b - 1)] = ax[:, r0:r1, t0:t1]
+            out += k[a, b] * src
+    return out
+
+
+def test_phase_contrast_forward_is_laplacian_of_absorption(scan_root):
+    """Every voxel's footprint is the Laplacian of its absorption footprint
+    trimmed to the detector, so Ax_pc = Laplacian(Ax) on the detector (a
+    transposed kernel or a dropped edge row fails this), and an interior voxel's
+    footprint sums to zero."""
+    cfg = dataclasses.replace(TINY, name="pc_pin")
+    path, info = make_scan(cfg, scan_root, "pc")
+    o = py_oracle.Oracle(True)
+    S, O, T = o.setup(oracle_path(path, o), cfg.cli_args())
+    try:
+        vol = info["phantom"].astype(np.float64)
+        o.set_object(O, vol)
+        H = o.detector_response(S, O, T)
+        ax = o.forward_project(S, O, H)
+        o.set_phase_contrast(True)
+        try:
+            axp = o.forward_project(S, O, H)
+            one = np.zeros_like(vol)
+            one[0, 2, 16, 15] = 1.0
+            o.set_object(O, one)
+            foot = o.forward_project(S, O, H)
+        finally:
+            o.set_phase_contrast(False)
+        want = _lap_filter(ax, S.delta_r)
+        np.testing.assert_allclose(axp, want, rtol=0, atol=1e-12 * np.abs(want).max())
+        assert abs(foot.sum()) <= 1e-12 * np.abs(foot).sum()
+        assert np.count_nonzero(foot[:, :, 1]) > 0 and np.count_nonzero(foot[:, :, 3]) > 0
+    finally:
+        o.free(S, O, T)

@@ -46,6 +46,8 @@ struct XtGeom {
   // ((t / bs) & 1) at position (t / (2 bs)) * bs + t % bs (bs slices per block)
   int pm, bs, H;
   int wide;  // A columns of more than XT_MAXA bins (voxels wider than 2 detector bins)
+  int pc;    // phase contrast: footprints are the detector Laplacian of the absorption ones
+  float lap[9];  // Sinogram.Lap_Kernel, [m + 1][n + 1] for bin offset m, row offset n
 };
 
 // Offset of sinogram entry (view p, bin r, row t) in the HBM layout.
@@ -1504,6 +1506,302 @@ __global__ void k_counts_sino(const uint16_t *__restrict__ counts, const float *
     const float temp = (float)log((double)(bright[(size_t)j * N_t + k] / w));
     weight[o] = w;
     proj[o] = (float)(bh * (double)temp * (double)temp + (double)temp);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Phase contrast (the reference's -DPHASE_CONTRAST_TOMOGRAPHY build).  A voxel's
+// footprint is the detector Laplacian of its absorption footprint trimmed to
+// the detector (F/XT_AMatrix.c:113-187), so by linearity A_pc x = Lap * (A x)
+// on the detector: mode 0 out64 = Lap * ax, mode 1 e = y - Lap * ax - d.
+__global__ void k_pc_laplace(XtGeom g, const double *__restrict__ ax, int mode,
+                             double *__restrict__ out64, const float *__restrict__ y,
+                             const float *__restrict__ d, float *__restrict__ e) {
+  const size_t n = (size_t)g.N_p * g.N_r * g.N_t;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % g.N_t), r = (int)((i / g.N_t) % g.N_r);
+    const size_t v = i / ((size_t)g.N_r * g.N_t);
+    double s = 0.0;
+#pragma unroll
+    for (int m = -1; m <= 1; m++)
+#pragma unroll
+      for (int k = -1; k <= 1; k++) {
+        const int rr = r + m, tt = t + k;
+        if (rr >= 0 && rr < g.N_r && tt >= 0 && tt < g.N_t)
+          s += (double)g.lap[(m + 1) * 3 + (k + 1)] * ax[(v * g.N_r + rr) * g.N_t + tt];
+      }
+    if (mode == 0) {
+      out64[i] = s;
+    } else {
+      const size_t idx = xt_sidx(g, (int)v, r, t);
+      e[idx] = (float)((double)y[idx] - s - (double)d[(size_t)r * g.N_t + t]);
+    }
+  }
+}
+
+// Laplacian footprint entry of a column (first candidate bin `first`, values
+// vals[0..XT_MAXA) times delta_t) at bin r and row offset dt from the voxel's
+// row: sum over m = -1, 0, 1 of Lap[m + 1][1 - dt] * line(r + m), in the
+// reference's order and float steps (compute_LapMatrix_4m_AMatrix; zero
+// entries of the line add nothing).
+__device__ __forceinline__ float xt_pc_entry(const XtGeom &g, const float *vals, int first, int r,
+                                             int dt) {
+#pragma clang fp contract(off)
+  float acc = 0.f;
+#pragma unroll
+  for (int m = -1; m <= 1; m++) {
+    const int k = r + m - first;
+    const float lv = (k >= 0 && k < XT_MAXA) ? vals[k] : 0.f;
+    acc += g.lap[(m + 1) * 3 + (1 - dt)] * lv;
+  }
+  return acc;
+}
+
+// Phase-contrast voxel update (F/XT_VoxUpdate.c:590-845).  A slice's footprint
+// covers detector rows t-1..t+1, which its neighbours in the line share, so a
+// line's slices are updated one after another in ascending order (as the
+// reference's slice loop), each with the workgroup's threads spread over the
+// frame's views: theta over the footprint, reduction, the qGGMRF surrogate
+// step on thread 0, e -= A * delta with atomics, and a wait before the next
+// slice reads e.  Lines, the z-block parity and NHICD lists as k_icd_lines.
+template <int ZL>
+__global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+  constexpr int NT = 256;
+  extern __shared__ float4 smem4[];
+  float4 *a_v4 = smem4;                          // [max_views] bins 0..3 (x delta_t)
+  float2 *a_v2 = (float2 *)(a_v4 + g.max_views);  // [max_views] bins 4..5
+  int *a_first = (int *)(a_v2 + g.max_views);     // [max_views] first candidate bin
+  float *red1 = (float *)(a_first + g.max_views);  // [NT]
+  float *red2 = red1 + NT;                        // [NT]
+  float *xn = red2 + NT;                          // [11][ZL]
+  float *magb = xn + 11 * ZL;                     // [64]
+  __shared__ float s_delta;
+  __shared__ int s_skip, s_cnt;
+  const int tid = threadIdx.x;
+  const int f = blockIdx.y;
+  const int nv = B.proj_num[f];
+  const int *vlist = B.views + (size_t)f * g.max_views;
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  const int iter = *B.iter;
+  const int blk_sz = g.N_z / g.nb;
+  const int par = (L.mode == 2) ? -1 : ((L.half ^ f) & 1);
+  auto zblock = [&](int z) {
+    int b = blk_sz > 0 ? z / blk_sz : g.nb - 1;
+    return b >= g.nb ? g.nb - 1 : b;
+  };
+  int lo = 0, hi = 0, n_dom = 0;
+  const int *list = B.lines;
+  uint32_t key = 0;
+  if (L.mode == 0) {
+    const int n = B.n_lines, per = n / L.K;
+    lo = L.sub_iter * per;
+    hi = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
+    n_dom = n;
+    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+  } else if (L.mode == 1) {
+    hi = B.nhicd_count[f * 2 + par];
+    list = B.nhicd_list + ((size_t)f * 2 + par) * NYX;
+  } else {
+    hi = B.n_lines;
+  }
+  int i0 = lo + (int)blockIdx.x, i1 = hi, istep = (int)gridDim.x;
+  if (L.mode == 1) {
+    const long long len = hi - lo;
+    i0 = lo + (int)(len * blockIdx.x / gridDim.x);
+    i1 = lo + (int)(len * (blockIdx.x + 1) / gridDim.x);
+    istep = 1;
+  }
+  const float T2 = g.T * g.T, Td = g.T * g.delta;
+  float vox_mag = 0.f;
+  double n_upd = 0.0, n_foot = 0.0, n_vis = 0.0;
+  for (int i = i0; i < i1; i += istep) {
+    const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
+    const int yy = c / g.N_x, xx = c - yy * g.N_x;
+    for (int p = tid; p < nv; p += NT) {
+      const int v = vlist[p];
+      int first = 0;
+      float vals[XT_MAXA];
+      xt_amatrix_column(g, B.Hr + (size_t)v * XT_HR_STRIDE, B.cosv[v], B.sinv[v], yy, xx, &first,
+                        vals);
+      a_v4[p] = make_float4(vals[0] * g.delta_t, vals[1] * g.delta_t, vals[2] * g.delta_t,
+                            vals[3] * g.delta_t);
+      a_v2[p] = make_float2(vals[4] * g.delta_t, vals[5] * g.delta_t);
+      a_first[p] = first;
+    }
+    for (int q = tid; q < 11 * ZL; q += NT) {
+      const int k = q / ZL, z = q - k * ZL;
+      float val = 0.f;
+      if (z < g.N_z) {
+        if (k < 9) {
+          const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+          if (y2 >= 0 && y2 < g.N_y && x2 >= 0 && x2 < g.N_x)
+            val = __hip_atomic_load(&B.x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z],
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          const int f2 = f + (k == 9 ? -1 : 1);
+          if (f2 >= 0 && f2 < g.N_time)
+            val = __hip_atomic_load(&B.x[((size_t)f2 * NYX + c) * g.N_z + z], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      xn[q] = val;
+    }
+    if (tid < 64) magb[tid] = 0.f;
+    __syncthreads();
+    float *xcol = xn + 4 * ZL;
+    for (int z = 0; z < g.N_z; z++) {
+      bool upd = par < 0 || (zblock(z) & 1) == par;
+      if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+      if (!upd) continue;  // uniform over the workgroup
+      // zero skipping, F/XT_VoxUpdate.c:805-826 (the 26 neighbours and two frames)
+      if (tid == 0) {
+        const float V = xcol[z];
+        bool any_pos = false;
+        for (int dz = -1; dz <= 1; dz++)
+          for (int k = 0; k < 9; k++) {
+            const int z2 = z + dz;
+            if ((dz == 0 && k == 4) || z2 < 0 || z2 >= g.N_z) continue;
+            const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+            if (y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x) continue;
+            any_pos |= xn[k * ZL + z2] > 0.f;
+          }
+        any_pos |= (f > 0 && xn[9 * ZL + z] > 0.f) || (f + 1 < g.N_time && xn[10 * ZL + z] > 0.f);
+        s_skip = (V == 0.f) && (iter > 1) && !any_pos;
+        s_cnt = 0;
+      }
+      __syncthreads();
+      const int skip = s_skip;
+      __syncthreads();  // every thread has read s_skip before the next slice rewrites it
+      if (skip) continue;
+      // theta over the footprint rows z-1..z+1, bins first-1..first+XT_MAXA
+      float th1 = 0.f, th2 = 0.f;
+      int cnt = 0;
+      for (int p = tid; p < nv; p += NT) {
+        const int v = vlist[p];
+        const float4 a4 = a_v4[p];
+        const float2 a2 = a_v2[p];
+        const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
+        const int first = a_first[p];
+        for (int dt = -1; dt <= 1; dt++) {
+          const int t = z + dt;
+          if (t < 0 || t >= g.N_t) continue;
+          for (int r = first - 1; r <= first + XT_MAXA; r++) {
+            if (r < 0 || r >= g.N_r) continue;
+            const float pe = xt_pc_entry(g, vals, first, r, dt);
+            if (pe == 0.f) continue;
+            const size_t idx = xt_sidx(g, v, r, t);
+            const float e1 = B.e[idx], w1 = B.w[idx];
+            const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
+            const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
+            const float pw = pe * ws;
+            th2 += pe * pw;
+            th1 -= e1 * pw;
+            cnt++;
+          }
+        }
+      }
+      red1[tid] = th1;
+      red2[tid] = th2;
+      if (cnt) atomicAdd(&s_cnt, cnt);
+      __syncthreads();
+      for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+          red1[tid] += red1[tid + st];
+          red2[tid] += red2[tid + st];
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
+        const float V = xcol[z];
+        float s1 = 0.f, s2 = 0.f;
+        for (int dz = -1; dz <= 1; dz++)
+          for (int k = 0; k < 9; k++) {
+            const int l = ((dz + 1) * 3 + k / 3) * 3 + k % 3;
+            const int z2 = z + dz, y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+            if (l == 13 || z2 < 0 || z2 >= g.N_z || y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x)
+              continue;
+            const float nb = xn[k * ZL + z2];
+            const float bq =
+                g.filt[l] * xt_qggmrf_ratio(V - nb, g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+            s1 += bq * nb;
+            s2 += bq;
+          }
+        if (f > 0) {
+          const float tn = xn[9 * ZL + z];
+          const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+          s1 += bq * tn;
+          s2 += bq;
+        }
+        if (f + 1 < g.N_time) {
+          const float tn = xn[10 * ZL + z];
+          const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+          s1 += bq * tn;
+          s2 += bq;
+        }
+        const float T1 = red1[0], T2z = red2[0];
+        const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
+        float nv2 = V + g.alpha * (u - V);
+        if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
+        xcol[z] = nv2;
+        s_delta = nv2 - V;
+        magb[zblock(z)] += fabsf(nv2 - V);
+        vox_mag += fabsf(nv2);
+        n_upd += 1.0;
+        n_foot += (double)s_cnt;
+      }
+      __syncthreads();
+      const float dl = s_delta;
+      if (dl != 0.f) {
+        for (int p = tid; p < nv; p += NT) {
+          const int v = vlist[p];
+          const float4 a4 = a_v4[p];
+          const float2 a2 = a_v2[p];
+          const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
+          const int first = a_first[p];
+          for (int dt = -1; dt <= 1; dt++) {
+            const int t = z + dt;
+            if (t < 0 || t >= g.N_t) continue;
+            for (int r = first - 1; r <= first + XT_MAXA; r++) {
+              if (r < 0 || r >= g.N_r) continue;
+              const float pe = xt_pc_entry(g, vals, first, r, dt);
+              if (pe != 0.f) atomicAdd(&B.e[xt_sidx(g, v, r, t)], -(pe * dl));
+            }
+          }
+        }
+      }
+      // the next slice's theta reads these e entries
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    // ---- write the line back and its update magnitude (as k_icd_lines)
+    if (tid < g.N_z)
+      __hip_atomic_store(&B.x[((size_t)f * NYX + c) * g.N_z + tid], xcol[tid], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < g.nb) {
+      const size_t mi = ((size_t)f * g.nb + tid) * NYX + c;
+      if ((par < 0 || (tid & 1) == par) && (L.mode != 1 || B.nhicd_blk[mi] != 0))
+        B.mag_blk[mi] = magb[tid];
+      else
+        magb[tid] = B.mag_blk[mi];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float sm = 0.f;
+      for (int b = 0; b < g.nb; b++) sm += magb[b];
+      B.magsum[(size_t)f * NYX + c] = sm;
+      n_vis += 1.0;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
+    if (n_vis != 0.0) {
+      atomicAdd(&B.stats[1], n_upd);
+      atomicAdd(&B.stats[2], n_foot);
+      atomicAdd(&B.stats[3], n_vis);
+    }
   }
 }
 

@@ -314,6 +314,12 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
     return fail(XT_E_ARG, "delta_xy/delta_z do not divide N_r/N_t");
   S->delta_r = S->Length_R / (S->N_r);
   S->delta_t = S->Length_T / (S->N_t);
+  {
+    // the detector Laplacian of phase-contrast footprints, F/XT_Init.c:284,439-441
+    const Real_t lap[3][3] = {{-0.5f, -1.f, -0.5f}, {-1.f, 6.f, -1.f}, {-0.5f, -1.f, -0.5f}};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) S->Lap_Kernel[i][j] = lap[i][j] / (S->delta_r * S->delta_r);
+  }
   S->R0 = -T->RotCenter * S->delta_r;
   S->RMax = (S->N_r - T->RotCenter) * S->delta_r;
   S->T0 = (float)(-S->Length_T / 2.0);

@@ -62,21 +62,21 @@ int xt_report_error(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[17] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0};  // by XT_OPT_*
+static double g_opt[18] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0};  // XT_OPT_*
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 16) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 17) return set_err(XT_E_ARG, "unknown option %d", option);
   if (option == XT_OPT_LINE_THREADS && value != 0 && value != 64 && value != 256)
     return set_err(XT_E_ARG, "XT_OPT_LINE_THREADS must be 0, 64 or 256");
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 16) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 17) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -145,6 +145,10 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
     return set_err(XT_E_ARG, "delta_xy/delta_r = %g too large for one A column (at most 6)",
                    (double)(O->delta_xy / S->delta_r));
   g->wide = 2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA;
+  g->pc = g_opt[XT_OPT_PHASE_CONTRAST] != 0;
+  if (g->pc && g->wide)
+    return set_err(XT_E_ARG, "phase contrast needs voxels at most 2 detector bins wide");
+  for (int i = 0; i < 9; i++) g->lap[i] = S->Lap_Kernel[i / 3][i % 3];
   if (T && T->node_num > 1)
     return set_err(XT_E_ARG, "node_num=%d: this engine runs one slab on one GPU", T->node_num);
   g->N_p = S->N_p;
@@ -243,6 +247,7 @@ struct xt_icd_state {
   float4 *at_vals = nullptr;  // A-column table (null: columns built on the fly)
   int *at_first = nullptr;
   int *iter_dev = nullptr;      // iteration number read by the voxel-update kernel
+  double *pc_tmp = nullptr;     // phase contrast: the absorption projection (FP64)
   hipGraph_t graph = nullptr;   // one captured iteration (iterations >= 2)
   hipGraphExec_t graph_exec = nullptr;
   int64_t graph_launches = 0;   // voxel-update launches in the captured iteration
@@ -266,7 +271,7 @@ static void free_state(xt_icd_state *s) {
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
-                  s->at_vals,  s->at_first,   s->iter_dev};
+                  s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *b : bufs)
@@ -296,14 +301,29 @@ static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const flo
                        x, Hr, c, s, vf, mode, out64, y, d, e);
 }
 
+// Ax (mode 0, FP64 into out64) or e = y - Ax - d (mode 1).  Phase contrast:
+// the absorption projection into pc_tmp (FP64, a sinogram's size), then its
+// detector Laplacian.
 static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *x, const float *Hr,
                           const float *c, const float *s, const int *vf, int mode, double *out64,
-                          const float *y, const float *d, float *e) {
+                          const float *y, const float *d, float *e, double *pc_tmp = nullptr) {
+  double *ax = out64;
+  int m = mode;
+  if (g.pc) {
+    if (!pc_tmp) return set_err(XT_E_ARG, "phase contrast without its projection buffer");
+    ax = pc_tmp;
+    m = 0;
+  }
   switch (ZL) {
-    case 8: fp_launch<8>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
-    case 16: fp_launch<16>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
-    case 32: fp_launch<32>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
-    default: fp_launch<64>(st, g, x, Hr, c, s, vf, mode, out64, y, d, e); break;
+    case 8: fp_launch<8>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
+    case 16: fp_launch<16>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
+    case 32: fp_launch<32>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
+    default: fp_launch<64>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
+  }
+  if (g.pc) {
+    const size_t n = (size_t)g.N_p * g.N_r * g.N_t;
+    hipLaunchKernelGGL(k_pc_laplace, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)),
+                       dim3(256), 0, st, g, pc_tmp, mode, out64, y, d, e);
   }
   XT_HIP(hipGetLastError());
   return XT_OK;
@@ -397,6 +417,19 @@ static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L, int gx) {
     e1 = s->ev[s->ev_used + 1];
     s->ev_used += 2;
     XT_HIP(hipEventRecord(e0, s->st));
+  }
+  if (s->g.pc) {
+    const dim3 grid(gx, s->g.N_time), block(256);
+    switch (s->ZL) {
+      case 8: hipLaunchKernelGGL(k_icd_lines_pc<8>, grid, block, lds, s->st, s->g, B, L); break;
+      case 16: hipLaunchKernelGGL(k_icd_lines_pc<16>, grid, block, lds, s->st, s->g, B, L); break;
+      case 32: hipLaunchKernelGGL(k_icd_lines_pc<32>, grid, block, lds, s->st, s->g, B, L); break;
+      default: hipLaunchKernelGGL(k_icd_lines_pc<64>, grid, block, lds, s->st, s->g, B, L); break;
+    }
+    XT_HIP(hipGetLastError());
+    if (s->timing) XT_HIP(hipEventRecord(e1, s->st));
+    s->launches++;
+    return XT_OK;
   }
   switch (s->ZL) {
     case 8: icd_launch<8>(s->st, s->g, B, L, gx, lds, s->nt); break;
@@ -624,7 +657,7 @@ static int init_device(int *dev) {
 
 static int icd_resync(xt_icd_state *s) {
   return launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 1,
-                        nullptr, s->y, s->d, s->e);
+                        nullptr, s->y, s->d, s->e, s->pc_tmp);
 }
 
 template <int ZL>
@@ -646,7 +679,7 @@ static int icd_autotune(xt_icd_state *s) {
   std::vector<std::pair<int, int>> cand;
   const int nt_opt = (int)g_opt[XT_OPT_LINE_THREADS];
   for (int nt : {256, 64}) {
-    if (nt_opt && nt != nt_opt) continue;
+    if ((nt_opt && nt != nt_opt) || (s->g.pc && nt != 256)) continue;
     int k = 0;
     for (int w = s->conc; w >= 16 && k < 3; w /= 2, k++) cand.push_back({w, nt});
   }
@@ -716,6 +749,7 @@ static int icd_autotune(xt_icd_state *s) {
 // so the rings go into d rather than into the starting object.
 static int icd_fbp(xt_icd_state *s, bool resync = true) {
   const XtGeom &g = s->g;
+  if (g.pc) return set_err(XT_E_ARG, "the filtered backprojection start is for absorption data");
   float *q = nullptr;
   XT_TRY(dev_alloc(&q, s->nsino));
   if (s->update_offset > 0) {
@@ -748,7 +782,7 @@ static int icd_fbp(xt_icd_state *s, bool resync = true) {
 // The A-column table is used when it fits in half of the free device memory
 // (XT_OPT_ATABLE = 1, the default) and every column spans at most four bins.
 static int build_atable(xt_icd_state *s) {
-  if (g_opt[XT_OPT_ATABLE] == 0 || s->g.wide) return XT_OK;
+  if (g_opt[XT_OPT_ATABLE] == 0 || s->g.wide || s->g.pc) return XT_OK;
   const XtGeom &g = s->g;
   const size_t n = (size_t)g.N_time * s->NYX * g.max_views;
   size_t free_b = 0, total_b = 0;
@@ -819,13 +853,14 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     }
   s->n_masked = (int)masked.size();
   // split the sinogram by z-block parity when the passes can pack their lanes
-  if (g_opt[XT_OPT_PACK] != 0 && g.nb % 2 == 0 && g.N_z % g.nb == 0 && g.N_t % 2 == 0) {
+  if (g_opt[XT_OPT_PACK] != 0 && !g.pc && g.nb % 2 == 0 && g.N_z % g.nb == 0 && g.N_t % 2 == 0) {
     g.pm = 1;
     g.bs = g.N_z / g.nb;
     g.H = g.N_t / 2;
   }
   if ((rc = setup_lines(s, masked))) return fail(rc);
   if ((rc = upload_views(s, S, O))) return fail(rc);
+  if (g.pc && (rc = dev_alloc(&s->pc_tmp, s->nsino))) return fail(rc);
   if ((rc = dev_alloc(&s->e, s->nsino)) || (rc = dev_alloc(&s->w, s->nsino)) ||
       (rc = dev_alloc(&s->y, s->nsino)) || (rc = dev_alloc(&s->d, (size_t)g.N_r * g.N_t)) ||
       (rc = dev_alloc(&s->x, s->nobj)) || (rc = dev_alloc(&s->Hr, (size_t)g.N_p * XT_HR_STRIDE)) ||
@@ -890,7 +925,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = icd_resync(s))) return fail(rc);
   if (hipStreamSynchronize(s->st) != hipSuccess)
     return fail(set_err(XT_E_HIP, "initialisation kernels failed"));
-  s->nt = (int)g_opt[XT_OPT_LINE_THREADS] == 64 ? 64 : 256;
+  s->nt = ((int)g_opt[XT_OPT_LINE_THREADS] == 64 && !g.pc) ? 64 : 256;
   if (g_opt[XT_OPT_AUTOTUNE] != 0 && g_opt[XT_OPT_LINES] == 0 && (rc = icd_autotune(s)))
     return fail(rc);
   *out = s;
@@ -1196,7 +1231,7 @@ static int batch_plan(xt_icd_batch *b) {
   if (g_opt[XT_OPT_BATCH_FUSE] == 0) return XT_OK;
   const xt_icd_state *s0 = b->s[0];
   for (xt_icd_state *s : b->s)
-    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide) return XT_OK;
+    if (s->K != s0->K || s->no_nhicd != s0->no_nhicd || s->g.wide || s->g.pc) return XT_OK;
   for (size_t i = 0; i < b->s.size(); i++) {
     xt_icd_state *s = b->s[i];
     XtIcdLaunch L0{};
@@ -1674,11 +1709,12 @@ static int seam_state(SeamCtx &c, Sinogram *S, ScannedObject *O, TomoInputs *T) 
   XT_TRY(make_geom(S, O, T, &c.s->g));
   XT_HIP(hipStreamCreate(&c.s->st));
   c.s->ZL = zl_for(c.s->g.N_z);
-  c.s->nt = (int)g_opt[XT_OPT_LINE_THREADS] == 64 ? 64 : 256;
+  c.s->nt = ((int)g_opt[XT_OPT_LINE_THREADS] == 64 && !c.s->g.pc) ? 64 : 256;
   c.s->nsino = (size_t)S->N_p * S->N_r * S->N_t;
   c.s->NYX = (size_t)O->N_y * O->N_x;
   c.s->nobj = (size_t)O->N_time * c.s->NYX * O->N_z;
   c.s->T = T;
+  if (c.s->g.pc) XT_TRY(dev_alloc(&c.s->pc_tmp, c.s->nsino));
   if (T) {
     c.s->var_est = T->var_est;
     c.s->zero_mean = T->EnforceZeroMeanOffset;
@@ -1812,7 +1848,7 @@ extern "C" int xt_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, d
   double *out = nullptr;
   XT_TRY(dev_alloc(&out, s->nsino));
   int rc = launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 0, out,
-                          nullptr, nullptr, nullptr);
+                          nullptr, nullptr, nullptr, s->pc_tmp);
   if (rc == XT_OK) {
     hipError_t e = hipStreamSynchronize(s->st);
     if (e == hipSuccess) e = hipMemcpy(AX64, out, s->nsino * sizeof(double), hipMemcpyDeviceToHost);

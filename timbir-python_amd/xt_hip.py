@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
- OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS) = range(1, 17)
+ OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS, OPT_PHASE_CONTRAST) = range(1, 18)
 
 # every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
