@@ -257,10 +257,21 @@ int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap);
  * listed voxel lines of one frame, one line at a time in list order, every
  * slice of a line (slices in red-black order), with E kept in FP32.
  * MagUpdate[N_y][N_x] receives the summed |change| of each updated line.
- * Returns the summed |new value| of performed updates in *total_vox_mag. */
+ * Returns the summed |new value| of performed updates in *total_vox_mag.
+ * Mask: null, or the voxel mask [N_time][N_y][N_x][N_z] (updateVoxels' Mask,
+ * F/XT_VoxUpdate.c:138): voxels where it is 0 are left alone. */
 int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, int32_t frame,
                     int32_t n_lines, const int32_t *x_list, const int32_t *y_list, int32_t Iter,
-                    Real_t *MagUpdate, double *total_vox_mag);
+                    Real_t *MagUpdate, double *total_vox_mag, const uint8_t *Mask);
+
+/* edge_detect, F/XT_ImageProc.c:83-149: the edge-only update mask
+ * Mask[N_time][N_y][N_x][N_z] (one contiguous block, as the reference's
+ * multialloc) of O->Object: per frame and slice a 5x5 Gaussian blur, the 3x3
+ * gradient pair, |gradient| > 0.0002 inside radius_obj, a 5x5 dilation.  Slice m
+ * is read from Object[n][m + 1] and the blur is normalised by its own sum (the
+ * reference indexes the object transposed and sums into an uninitialised
+ * variable).  ICD_BackProject applies it when only_Edge_Updates is set. */
+int xt_edge_detect(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
 
 #ifdef __cplusplus
 }

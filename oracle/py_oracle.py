@@ -101,6 +101,10 @@ class Oracle:
         L.oracle_update_lines.argtypes = [P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, P, P, C.c_int32, P, P, C.c_int32]
         L.oracle_update_lines.restype = self.R
+        L.oracle_update_lines_mask.argtypes = [P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_int32, P, P, C.c_int32, P, P, P, C.c_int32]
+        L.oracle_update_lines_mask.restype = self.R
+        L.oracle_edge_detect.argtypes = [P, P, P]
         L.oracle_vsc_select.argtypes = [P, P, P]
         L.oracle_random_select.argtypes = [P, P, P]
         L.oracle_icd_backproject.argtypes = [P, P, P, C.c_char_p, C.c_int32, P]
@@ -131,6 +135,13 @@ class Oracle:
         out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), self.np_real)
         self.lib.oracle_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data)
         return out
+
+    def edge_detect(self, O, T):
+        """oracle_edge_detect: the edge-only mask [N_time][N_y][N_x][N_z] (uint8) of O."""
+        mask = np.zeros((O.N_time, O.N_y, O.N_x, O.N_z), np.uint8)
+        mt = xt_abi.PtrTree(mask)
+        self.lib.oracle_edge_detect(C.byref(O), C.byref(T), mt.ptr)
+        return mask
 
     def set_phase_contrast(self, on):
         """The -DPHASE_CONTRAST_TOMOGRAPHY build's Laplacian footprints on/off."""

@@ -1100,6 +1100,112 @@ static int32_t read_bin(const char *dir, const char *name, void *data, size_t n,
   return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Edge-only updates, F/XT_ImageProc.c:40-170.  filter_image (:40-55): out[i][j]
+ * = sum over k, then l, of h[k + r][l + r] * in[i + k][j + l] inside the image,
+ * r = (len - 1) / 2, in Real_t from 0. */
+static void edge_filter(Real_t *in, Real_t *out, Real_t h[5][5], int32_t rows, int32_t cols,
+                        int32_t len) {
+  int32_t i, j, k, l, r = (len - 1) / 2;
+  for (i = 0; i < rows; i++)
+    for (j = 0; j < cols; j++) {
+      Real_t acc = 0;
+      for (k = -r; k <= r; k++)
+        for (l = -r; l <= r; l++)
+          if (i + k >= 0 && j + l >= 0 && i + k < rows && j + l < cols)
+            acc += h[k + r][l + r] * in[(i + k) * cols + j + l];
+      out[i * cols + j] = acc;
+    }
+}
+
+/* edge_detect, F/XT_ImageProc.c:83-149: per frame and slice, a 5x5 Gaussian
+ * blur (variance 3), the 3x3 gradient pair hx, hy, |gradient| > 0.0002 inside
+ * radius_obj (centre from -Length/2), then a 5x5 dilation (image_dilation,
+ * :57-81); Mask[n][row][col][slice] = the dilated map.  Two departures make it
+ * a defined function: the blur's normaliser starts at 0 (the reference adds
+ * into an uninitialised `sum`, :95-100) and slice m is read as
+ * Object[n][m + 1][row][col] (the reference reads Object[n][row][col][m],
+ * :117, which leaves the object's N_z + 2 planes for N_y > N_z + 2). */
+void oracle_edge_detect(ScannedObject *O, TomoInputs *T, uint8_t ****Mask) {
+  Real_t hx[5][5] = {{3, 0, -3, 0, 0}, {10, 0, -10, 0, 0}, {3, 0, -3, 0, 0}};
+  Real_t hy[5][5] = {{3, 10, 3, 0, 0}, {0, 0, 0, 0, 0}, {-3, -10, -3, 0, 0}};
+  Real_t hblur[5][5], var = 3.0, sum = 0, thresh = 0.0002;
+  int32_t i, j, n, m, rows = O->N_y, cols = O->N_x;
+  for (i = -2; i <= 2; i++)
+    for (j = -2; j <= 2; j++) {
+      hblur[i + 2][j + 2] = exp(-(i * i + j * j) / (2 * var));
+      sum += hblur[i + 2][j + 2];
+    }
+  for (i = 0; i < 5; i++)
+    for (j = 0; j < 5; j++) hblur[i][j] /= sum;
+  size_t np = (size_t)rows * cols;
+  Real_t *img = (Real_t *)o_calloc(np, sizeof(Real_t));
+  Real_t *blur = (Real_t *)o_calloc(np, sizeof(Real_t));
+  Real_t *gx = (Real_t *)o_calloc(np, sizeof(Real_t));
+  Real_t *gy = (Real_t *)o_calloc(np, sizeof(Real_t));
+  uint8_t *edge = (uint8_t *)o_calloc(np, 1);
+  for (n = 0; n < O->N_time; n++)
+    for (m = 0; m < O->N_z; m++) {
+      for (i = 0; i < rows; i++)
+        for (j = 0; j < cols; j++) img[i * cols + j] = O->Object[n][m + 1][i][j];
+      edge_filter(img, blur, hblur, rows, cols, 5);
+      edge_filter(blur, gx, hx, rows, cols, 3);
+      edge_filter(blur, gy, hy, rows, cols, 3);
+      for (i = 0; i < rows; i++)
+        for (j = 0; j < cols; j++) {
+          Real_t x = -O->Length_X / 2 + ((Real_t)j + 0.5) * O->delta_xy;
+          Real_t y = -O->Length_Y / 2 + ((Real_t)i + 0.5) * O->delta_xy;
+          edge[i * cols + j] = 0;
+          if (x * x + y * y < T->radius_obj * T->radius_obj) {
+            Real_t gxv = gx[i * cols + j], gyv = gy[i * cols + j];
+            Real_t mag = sqrt(gxv * gxv + gyv * gyv);
+            if (mag > thresh) edge[i * cols + j] = 1;
+          }
+        }
+      for (i = 0; i < rows; i++)
+        for (j = 0; j < cols; j++) {
+          uint8_t v = 0;
+          for (int32_t k = -2; k <= 2; k++)
+            for (int32_t l = -2; l <= 2; l++)
+              if (i + k >= 0 && j + l >= 0 && i + k < rows && j + l < cols)
+                v |= edge[(i + k) * cols + j + l];
+          Mask[n][i][j][m] = v;
+        }
+    }
+  free(img);
+  free(blur);
+  free(gx);
+  free(gy);
+  free(edge);
+}
+
+/* select_edge_voxels, F/XT_ImageProc.c:151-170: the mask, and MagUpdateMap
+ * cleared for every (frame, z-block, line) without a masked voxel. */
+static void oracle_select_edge_voxels(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap,
+                                      uint8_t ****Mask) {
+  int32_t i, j, k, l, n, flag, bs = O->N_z / T->num_z_blocks;
+  oracle_edge_detect(O, T, Mask);
+  for (i = 0; i < O->N_time; i++)
+    for (k = 0; k < O->N_y; k++)
+      for (l = 0; l < O->N_x; l++)
+        for (j = 0; j < T->num_z_blocks; j++) {
+          flag = 0;
+          for (n = j * bs; n < (j + 1) * bs; n++)
+            if (Mask[i][k][l][n] == 1) flag = 1;
+          if (flag == 0) MagUpdateMap[i][j][k][l] = 0;
+        }
+}
+
+/* oracle_update_lines with the voxel mask of the frame ([N_y][N_x][N_z]). */
+Real_t oracle_update_lines_mask(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                                Real_t **H_r, int32_t frame, int32_t z0, int32_t z1,
+                                int32_t n_lines, const int32_t *x_list, const int32_t *y_list,
+                                int32_t Iter, int64_t *zero_count, Real_t **MagUpdateMap,
+                                uint8_t ***Mask, int32_t positivity) {
+  return update_lines_masked(S, O, T, E, H_r, frame, z0, z1, n_lines, x_list, y_list, Iter,
+                             zero_count, MagUpdateMap, Mask, positivity);
+}
+
 /* ICD driver, F/XT_ICD_update.c:928-1094.  initICD 0 (zero start), 1 (read
  * object_n0_time_<i>.bin), 2 and 3 (the previous stage's half-size object
  * upsampled, oracle_upsample_object). */
@@ -1153,6 +1259,8 @@ int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, con
     dbg(T, "ERROR: initObject: initICD value not recognized\n");
     return -1;
   }
+  /* F/XT_ICD_update.c:996-997 */
+  if (T->only_Edge_Updates == 1) oracle_select_edge_voxels(O, T, MagUpdateMap, Mask);
   oracle_init_error_sino(S, O, T, H_r, E);
   oracle_multifree(S->Projection, 3);
   S->Projection = NULL;

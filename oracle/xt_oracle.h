@@ -71,6 +71,13 @@ Real_t oracle_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t 
                            int32_t frame, int32_t z0, int32_t z1, int32_t n_lines,
                            const int32_t *x_list, const int32_t *y_list, int32_t Iter,
                            int64_t *zero_count, Real_t **MagUpdateMap, int32_t positivity);
+Real_t oracle_update_lines_mask(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                                Real_t **H_r, int32_t frame, int32_t z0, int32_t z1,
+                                int32_t n_lines, const int32_t *x_list, const int32_t *y_list,
+                                int32_t Iter, int64_t *zero_count, Real_t **MagUpdateMap,
+                                uint8_t ***Mask, int32_t positivity);
+/* Edge-only update mask [N_time][N_y][N_x][N_z] of the current object. */
+void oracle_edge_detect(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
 void oracle_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpdateMap);
 void oracle_random_select(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
 

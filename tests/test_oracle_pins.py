@@ -613,3 +613,69 @@ def test_phase_contrast_forward_is_laplacian_of_absorption(scan_root):
         assert np.count_nonzero(foot[:, :, 1]) > 0 and np.count_nonzero(foot[:, :, 3]) > 0
     finally:
         o.free(S, O, T)
+
+
+def _edge_mask_ndimage(img, length_x, delta_xy, radius, band):
+    """The edge-only mask of one slice with scipy.ndimage in float64: pixels
+    whose |gradient| is within `band` (relative) of the threshold are returned
+    as undecided (lo = 0, hi = 1) and so are the dilated pixels they reach."""
+    from scipy import ndimage
+    k = np.arange(-2, 3)
+    hb = np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / 6.0)
+    hb /= hb.sum()
+    hx = np.array([[3, 0, -3], [10, 0, -10], [3, 0, -3]], np.float64)
+    hy = np.array([[3, 10, 3], [0, 0, 0], [-3, -10, -3]], np.float64)
+    blur = ndimage.correlate(img, hb, mode="constant", cval=0.0)
+    mag = np.hypot(ndimage.correlate(blur, hx, mode="constant"),
+                   ndimage.correlate(blur, hy, mode="constant"))
+    n_y, n_x = img.shape
+    xc = -length_x / 2 + (np.arange(n_x) + 0.5) * delta_xy
+    yc = -length_x / 2 + (np.arange(n_y) + 0.5) * delta_xy
+    inside = (xc[None, :] ** 2 + yc[:, None] ** 2) < radius ** 2
+    thr = 0.0002
+    lo = inside & (mag > thr * (1 + band))
+    hi = inside & (mag > thr * (1 - band))
+    sq = np.ones((5, 5), bool)
+    return (ndimage.binary_dilation(lo, structure=sq).astype(np.uint8),
+            ndimage.binary_dilation(hi, structure=sq).astype(np.uint8), mag)
+
+
+def test_edge_detect_matches_ndimage(scan_root):
+    """oracle_edge_detect (F/XT_ImageProc.c:83-149) against scipy.ndimage on the
+    same steps in float64: a normalised 5x5 Gaussian (variance 3), the 3x3
+    gradient pair, |gradient| > 0.0002 inside radius_obj, a 5x5 dilation.  The
+    slices are smooth random fields scaled so that about half their pixels lie
+    above the threshold, so a missing normaliser, another variance or a 3x3
+    dilation moves many pixels; each slice has its own field, so reading the
+    wrong plane fails.  Pixels within 0.1% of the threshold are left open."""
+    from scipy import ndimage
+    cfg = dataclasses.replace(TINY, name="edge_pin")
+    path, _ = make_scan(cfg, scan_root, "edge")
+    o = py_oracle.Oracle(False)
+    S, O, T = o.setup(path, cfg.cli_args())
+    try:
+        T.radius_obj = 12.0
+        rng = np.random.default_rng(5)
+        vol = np.zeros((O.N_time, O.N_z, O.N_y, O.N_x))
+        for m in range(O.N_z):
+            f = ndimage.gaussian_filter(rng.normal(size=(O.N_y, O.N_x)), 1.5 + 0.5 * m)
+            _, _, mag = _edge_mask_ndimage(f, O.Length_X, O.delta_xy, 1e9, 0.0)
+            vol[0, m] = f * (0.0002 / np.median(mag))
+        vol = vol.astype(np.float32)
+        o.set_object(O, vol)
+        got = o.edge_detect(O, T)
+        n_open = 0
+        for m in range(O.N_z):
+            lo, hi, _ = _edge_mask_ndimage(vol[0, m].astype(np.float64), O.Length_X, O.delta_xy,
+                                           T.radius_obj, 1e-3)
+            g = got[0, :, :, m]
+            assert np.all(g >= lo) and np.all(g <= hi), f"slice {m}"
+            n_open += int((lo != hi).sum())
+            assert 0 < g.sum() < g.size
+        assert n_open < 0.02 * got.size
+        # outside the circle grown by the 2-pixel dilation nothing is set
+        xc = -O.Length_X / 2 + (np.arange(O.N_x) + 0.5) * O.delta_xy
+        r = np.hypot(xc[None, :], xc[:, None])
+        assert not got[0][r > T.radius_obj + 3 * O.delta_xy].any()
+    finally:
+        o.free(S, O, T)

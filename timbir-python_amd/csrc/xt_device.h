@@ -478,6 +478,8 @@ struct XtIcdBuffers {
   const int *at_first;       // first bin of each table entry
   const int *iter;           // ICD iteration number (device memory, so a captured
                              // graph of one iteration replays for every iteration)
+  const uint8_t *vmask;      // [N_time][N_y][N_x][N_z] voxels to update (edge-only
+                             // updates), or null: every voxel of a listed line
 };
 
 // ---------------------------------------------------------------------------
@@ -846,6 +848,7 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
       if (z < g.N_z && mine) {
         upd = par < 0 || (zblock(z) & 1) == par;
         if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+        if (upd && B.vmask) upd = B.vmask[((size_t)f * NYX + c) * g.N_z + z] != 0;
       }
       if (upd) {
         const float V = xcol[z];
@@ -1653,6 +1656,7 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
     for (int z = 0; z < g.N_z; z++) {
       bool upd = par < 0 || (zblock(z) & 1) == par;
       if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+      if (upd && B.vmask) upd = B.vmask[((size_t)f * NYX + c) * g.N_z + z] != 0;
       if (!upd) continue;  // uniform over the workgroup
       // zero skipping, F/XT_VoxUpdate.c:805-826 (the 26 neighbours and two frames)
       if (tid == 0) {
@@ -1802,6 +1806,102 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
       atomicAdd(&B.stats[2], n_foot);
       atomicAdd(&B.stats[3], n_vis);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Edge-only updates (F/XT_ImageProc.c:40-149): the voxel update mask of the
+// current object, per frame and slice: a 5x5 Gaussian blur, the 3x3 gradient
+// pair, |gradient| > 0.0002 inside the object circle, then a 5x5 dilation.
+// Each sum runs in the reference's order in float with contraction off, so the
+// mask is bit-exact.  Threads run over the device layout [N_time][N_y][N_x][N_z]
+// (slices fastest: neighbouring threads read neighbouring words).
+struct XtEdgeParams {
+  float hblur[25];  // normalised blur weights [k + 2][l + 2], row offset k, column offset l
+  float half_x, half_y;  // -Length_X / 2, -Length_Y / 2
+  float delta_xy;
+  float r2;  // radius_obj * radius_obj
+};
+
+__global__ void k_edge_blur(XtGeom g, XtEdgeParams P, const float *__restrict__ x,
+                            float *__restrict__ blur) {
+#pragma clang fp contract(off)
+  const size_t NYX = (size_t)g.N_y * g.N_x, n = (size_t)g.N_time * NYX * g.N_z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % g.N_z);
+    const size_t fc = i / g.N_z;
+    const int c = (int)(fc % NYX), row = c / g.N_x, col = c - row * g.N_x;
+    const size_t base = (fc - c) * g.N_z;  // frame start
+    float acc = 0.f;
+    for (int k = -2; k <= 2; k++) {
+      const int r2 = row + k;
+      if (r2 < 0 || r2 >= g.N_y) continue;
+      for (int l = -2; l <= 2; l++) {
+        const int c2 = col + l;
+        if (c2 < 0 || c2 >= g.N_x) continue;
+        acc += P.hblur[(k + 2) * 5 + (l + 2)] * x[base + ((size_t)r2 * g.N_x + c2) * g.N_z + z];
+      }
+    }
+    blur[i] = acc;
+  }
+}
+
+__global__ void k_edge_gradient(XtGeom g, XtEdgeParams P, const float *__restrict__ blur,
+                                uint8_t *__restrict__ edge) {
+#pragma clang fp contract(off)
+  const float hx[9] = {3.f, 0.f, -3.f, 10.f, 0.f, -10.f, 3.f, 0.f, -3.f};
+  const float hy[9] = {3.f, 10.f, 3.f, 0.f, 0.f, 0.f, -3.f, -10.f, -3.f};
+  const size_t NYX = (size_t)g.N_y * g.N_x, n = (size_t)g.N_time * NYX * g.N_z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % g.N_z);
+    const size_t fc = i / g.N_z;
+    const int c = (int)(fc % NYX), row = c / g.N_x, col = c - row * g.N_x;
+    const size_t base = (fc - c) * g.N_z;
+    float gx = 0.f, gy = 0.f;
+    for (int k = -1; k <= 1; k++) {
+      const int r2 = row + k;
+      if (r2 < 0 || r2 >= g.N_y) continue;
+      for (int l = -1; l <= 1; l++) {
+        const int c2 = col + l;
+        if (c2 < 0 || c2 >= g.N_x) continue;
+        const float v = blur[base + ((size_t)r2 * g.N_x + c2) * g.N_z + z];
+        gx += hx[(k + 1) * 3 + (l + 1)] * v;
+        gy += hy[(k + 1) * 3 + (l + 1)] * v;
+      }
+    }
+    // the circle of F/XT_ImageProc.c:122-124: centre from -Length / 2, double sums
+    const float xc = (float)((double)P.half_x + ((double)(float)col + 0.5) * (double)P.delta_xy);
+    const float yc = (float)((double)P.half_y + ((double)(float)row + 0.5) * (double)P.delta_xy);
+    uint8_t on = 0;
+    if (xc * xc + yc * yc < P.r2) {
+      const float mag = (float)sqrt((double)(gx * gx + gy * gy));
+      on = mag > 0.0002f;
+    }
+    edge[i] = on;
+  }
+}
+
+__global__ void k_edge_dilate(XtGeom g, const uint8_t *__restrict__ edge,
+                              uint8_t *__restrict__ mask) {
+  const size_t NYX = (size_t)g.N_y * g.N_x, n = (size_t)g.N_time * NYX * g.N_z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % g.N_z);
+    const size_t fc = i / g.N_z;
+    const int c = (int)(fc % NYX), row = c / g.N_x, col = c - row * g.N_x;
+    const size_t base = (fc - c) * g.N_z;
+    uint8_t v = 0;
+    for (int k = -2; k <= 2; k++) {
+      const int r2 = row + k;
+      if (r2 < 0 || r2 >= g.N_y) continue;
+      for (int l = -2; l <= 2; l++) {
+        const int c2 = col + l;
+        if (c2 >= 0 && c2 < g.N_x) v |= edge[base + ((size_t)r2 * g.N_x + c2) * g.N_z + z];
+      }
+    }
+    mask[i] = v;
   }
 }
 

@@ -187,8 +187,8 @@ int parse_args(int argc, char **argv, Sinogram *S, ScannedObject *O, TomoInputs 
   if (T->sinobin != 1)
     return fail(XT_E_ARG, "sinobin=%d: only projection/weight .bin inputs (sinobin 1) are read",
                 (int)T->sinobin);
-  if (T->only_Edge_Updates || T->readSino4mHDF)
-    return fail(XT_E_ARG, "edge-only updates and HDF5 input are not part of this engine");
+  if (T->readSino4mHDF)
+    return fail(XT_E_ARG, "HDF5 input is read by the ingest module, not by the engine");
   O->delta_xy = O->mult_xy;  // F/XT_Init.c:617-622
   O->delta_z = O->mult_z;
   S->z_overlap_num = (int32_t)O->mult_z;

@@ -248,6 +248,9 @@ struct xt_icd_state {
   int *at_first = nullptr;
   int *iter_dev = nullptr;      // iteration number read by the voxel-update kernel
   double *pc_tmp = nullptr;     // phase contrast: the absorption projection (FP64)
+  int edge = 0;                 // edge-only updates (only_Edge_Updates)
+  XtEdgeParams edge_par{};
+  uint8_t *vmask = nullptr;     // edge-only: [N_time][N_y][N_x][N_z] voxels to update
   hipGraph_t graph = nullptr;   // one captured iteration (iterations >= 2)
   hipGraphExec_t graph_exec = nullptr;
   int64_t graph_launches = 0;   // voxel-update launches in the captured iteration
@@ -271,7 +274,7 @@ static void free_state(xt_icd_state *s) {
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
-                  s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp};
+                  s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp, s->vmask};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *b : bufs)
@@ -396,6 +399,7 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   B.at_vals = s->at_vals;
   B.at_first = s->at_first;
   B.iter = s->iter_dev;
+  B.vmask = s->vmask;
   return B;
 }
 
@@ -747,6 +751,8 @@ static int icd_autotune(xt_icd_state *s) {
 // x = filtered backprojection of y - d, then e = y - Ax - d.  When the ring
 // offsets are estimated, d first gets a guess from the view-averaged sinogram,
 // so the rings go into d rather than into the starting object.
+static int select_edges(xt_icd_state *s);
+
 static int icd_fbp(xt_icd_state *s, bool resync = true) {
   const XtGeom &g = s->g;
   if (g.pc) return set_err(XT_E_ARG, "the filtered backprojection start is for absorption data");
@@ -776,6 +782,7 @@ static int icd_fbp(xt_icd_state *s, bool resync = true) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->st);
   (void)hipFree(q);
   if (e != hipSuccess) return set_err(XT_E_HIP, "FBP: %s", hipGetErrorString(e));
+  if (s->edge && resync) XT_TRY(select_edges(s));
   return resync ? icd_resync(s) : XT_OK;
 }
 
@@ -811,6 +818,94 @@ static int build_atable(xt_icd_state *s) {
   return XT_OK;
 }
 
+// Edge-detection constants of F/XT_ImageProc.c:85-101,122-124: the 5x5 Gaussian
+// (variance 3) in float, normalised by its float sum, and the object circle.
+static XtEdgeParams edge_params(const ScannedObject *O, const TomoInputs *T) {
+  XtEdgeParams P{};
+  const float var = 3.0f;
+  float sum = 0.f;
+  for (int i = -2; i <= 2; i++)
+    for (int j = -2; j <= 2; j++) {
+      P.hblur[(i + 2) * 5 + (j + 2)] = (float)exp((double)(-(i * i + j * j) / (2 * var)));
+      sum += P.hblur[(i + 2) * 5 + (j + 2)];
+    }
+  for (int k = 0; k < 25; k++) P.hblur[k] /= sum;
+  P.half_x = -O->Length_X / 2;
+  P.half_y = -O->Length_Y / 2;
+  P.delta_xy = O->delta_xy;
+  P.r2 = T->radius_obj * T->radius_obj;
+  return P;
+}
+
+// The edge-only voxel mask of the object x into mask (device layout, which is
+// also the reference's Mask[N_time][N_y][N_x][N_z]).
+static int launch_edge_mask(xt_icd_state *s, const float *x, uint8_t *mask) {
+  float *blur = nullptr;
+  uint8_t *edge = nullptr;
+  XT_TRY(dev_alloc(&blur, s->nobj));
+  int rc = dev_alloc(&edge, s->nobj);
+  if (rc == XT_OK) {
+    const dim3 grid(grid_1d(s->nobj)), block(256);
+    hipLaunchKernelGGL(k_edge_blur, grid, block, 0, s->st, s->g, s->edge_par, x, blur);
+    hipLaunchKernelGGL(k_edge_gradient, grid, block, 0, s->st, s->g, s->edge_par, blur, edge);
+    hipLaunchKernelGGL(k_edge_dilate, grid, block, 0, s->st, s->g, edge, mask);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) rc = set_err(XT_E_HIP, "edge mask: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(blur);
+  if (edge) (void)hipFree(edge);
+  return rc;
+}
+
+static void drop_graph(xt_icd_state *s) {
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+}
+
+// Edge-only updates, F/XT_ICD_update.c:996-997 with F/XT_ImageProc.c:151-170 and
+// the line selection of F/XT_ICD_update.c:599-631: the voxel mask of the current
+// object; the line list becomes every line with a masked voxel (in any frame),
+// and each (frame, z-block)'s NHICD share is its own count of such lines over
+// NHICD_Iterations.  The line kernel skips unmasked voxels.
+static int select_edges(xt_icd_state *s) {
+  const XtGeom &g = s->g;
+  if (!s->vmask) {
+    XT_TRY(dev_alloc(&s->vmask, s->nobj));
+    // the list may grow beyond the radius mask (the dilation reaches outside it)
+    if (s->lines) (void)hipFree(s->lines);
+    s->lines = nullptr;
+    XT_TRY(dev_alloc(&s->lines, s->NYX));
+  }
+  XT_TRY(launch_edge_mask(s, s->x, s->vmask));
+  std::vector<uint8_t> m(s->nobj);
+  XT_HIP(hipMemcpy(m.data(), s->vmask, s->nobj, hipMemcpyDeviceToHost));
+  const int nb = g.nb, bs = g.N_z / g.nb;
+  std::vector<int> lines, ns((size_t)g.N_time * (nb > 2 ? nb : 2), 0);
+  for (size_t c = 0; c < s->NYX; c++) {
+    bool any = false;
+    for (int f = 0; f < g.N_time; f++) {
+      const uint8_t *v = &m[((size_t)f * s->NYX + c) * g.N_z];
+      for (int z = 0; z < g.N_z; z++) any |= v[z] != 0;
+      for (int b = 0; b < nb; b++) {
+        bool has = false;
+        for (int z = b * bs; z < (b + 1) * bs; z++) has |= v[z] != 0;
+        ns[(size_t)f * nb + b] += has;
+      }
+    }
+    if (any) lines.push_back((int)c);
+  }
+  for (int &n : ns) n /= s->K;
+  s->n_masked = (int)lines.size();
+  if (!lines.empty())
+    XT_HIP(hipMemcpy(s->lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice));
+  drop_graph(s);
+  return XT_OK;
+}
+
 static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **out) {
   *out = nullptr;
   xt_icd_state *s = new xt_icd_state();
@@ -828,6 +923,8 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   s->T = T;
   s->K = O->NHICD_Iterations > 0 ? O->NHICD_Iterations : 1;
   s->NumIter = T->NumIter;
+  s->edge = T->only_Edge_Updates != 0;
+  s->edge_par = edge_params(O, T);
   s->update_offset = T->updateProjOffset;
   s->zero_mean = T->EnforceZeroMeanOffset;
   s->update_var = T->updateVar;
@@ -911,6 +1008,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       return fail(set_err(XT_E_HIP, "nsel upload"));
   }
   if ((rc = upload_object(s, O, s->x))) return fail(rc);
+  if (s->edge && (rc = select_edges(s))) return fail(rc);
   if ((rc = launch_detector_response(s, O->BeamWidth))) return fail(rc);
   if ((rc = build_atable(s))) return fail(rc);
   {
@@ -1121,6 +1219,7 @@ extern "C" int xt_icd_upload(xt_icd_state *s, Sinogram *S, ScannedObject *O, Tom
       O->N_x != s->g.N_x || S->N_r != s->g.N_r || S->N_t != s->g.N_t)
     return set_err(XT_E_ARG, "structs do not match the state's sizes");
   XT_TRY(upload_object(s, O, s->x));
+  if (s->edge) XT_TRY(select_edges(s));
   XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)s->g.N_r * s->g.N_t * sizeof(float),
                    hipMemcpyHostToDevice));
   XT_TRY(icd_resync(s));
@@ -1142,6 +1241,7 @@ extern "C" int xt_icd_unet_start(xt_icd_state *s, xt_unet *net) {
   if (g.positivity)
     hipLaunchKernelGGL(k_clamp_nonneg, dim3(1024), dim3(256), 0, s->st, s->x, s->nobj);
   XT_HIP(hipGetLastError());
+  if (s->edge) XT_TRY(select_edges(s));
   XT_TRY(icd_resync(s));
   XT_HIP(hipStreamSynchronize(s->st));
   return XT_OK;
@@ -1740,6 +1840,30 @@ static int seam_download_sel(xt_icd_state *s, Sinogram *S) {
   return XT_OK;
 }
 
+extern "C" int xt_edge_detect(ScannedObject *O, TomoInputs *T, uint8_t ****Mask) {
+  if (!O || !T || !Mask) return set_err(XT_E_ARG, "null argument");
+  if (O->N_time < 1 || O->N_y < 1 || O->N_x < 1 || O->N_z < 1)
+    return set_err(XT_E_ARG, "empty object");
+  SeamCtx c;
+  c.s = new xt_icd_state();
+  xt_icd_state *s = c.s;
+  XT_TRY(init_device(&s->dev));
+  XT_HIP(hipStreamCreate(&s->st));
+  s->g.N_time = O->N_time;
+  s->g.N_y = O->N_y;
+  s->g.N_x = O->N_x;
+  s->g.N_z = O->N_z;
+  s->NYX = (size_t)O->N_y * O->N_x;
+  s->nobj = (size_t)O->N_time * s->NYX * O->N_z;
+  s->edge_par = edge_params(O, T);
+  XT_TRY(dev_alloc(&s->x, s->nobj));
+  XT_TRY(dev_alloc(&s->vmask, s->nobj));
+  XT_TRY(upload_object(s, O, s->x));
+  XT_TRY(launch_edge_mask(s, s->x, s->vmask));
+  XT_HIP(hipMemcpy(&Mask[0][0][0][0], s->vmask, s->nobj, hipMemcpyDeviceToHost));
+  return XT_OK;
+}
+
 extern "C" int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t **H_r) {
   if (!S || !O || !H_r) return set_err(XT_E_ARG, "null argument");
   SeamCtx c;
@@ -2004,7 +2128,7 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
 extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
                                int32_t frame, int32_t n_lines, const int32_t *x_list,
                                const int32_t *y_list, int32_t Iter, Real_t *MagUpdate,
-                               double *total_vox_mag) {
+                               double *total_vox_mag, const uint8_t *Mask) {
   if (!S || !O || !T || !E || n_lines < 0 || (n_lines && (!x_list || !y_list)))
     return set_err(XT_E_ARG, "bad argument");
   if (frame < 0 || frame >= O->N_time) return set_err(XT_E_ARG, "frame %d", frame);
@@ -2034,6 +2158,10 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(upload_object(s, O, s->x));
   XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
   XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  if (Mask) {
+    XT_TRY(dev_alloc(&s->vmask, s->nobj));
+    XT_HIP(hipMemcpy(s->vmask, Mask, s->nobj, hipMemcpyHostToDevice));
+  }
   XT_TRY(dev_alloc(&s->nhicd_flag, (size_t)g.N_time * s->NYX));
   XT_HIP(hipMemset(s->nhicd_flag, 1, (size_t)g.N_time * s->NYX));
   XT_TRY(dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX));

@@ -31,7 +31,7 @@ EXPORTS = [
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
-    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines",
+    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines", "xt_edge_detect",
     "xt_unet_create", "xt_unet_destroy", "xt_unet_param_count", "xt_unet_get_params",
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
@@ -513,14 +513,32 @@ def vsc_select(eng: Engine, mag: np.ndarray, update_select_num: np.ndarray):
             for i in range(O.N_time)]
 
 
-def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int):
-    """Sequential update of the listed lines; returns (MagUpdate [N_y][N_x], total_vox_mag)."""
+def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int, mask=None):
+    """Sequential update of the listed lines; returns (MagUpdate [N_y][N_x], total_vox_mag).
+    mask: optional uint8 voxel mask [N_time][N_y][N_x][N_z] (0 = leave the voxel)."""
     xs = np.ascontiguousarray(xs, np.int32)
     ys = np.ascontiguousarray(ys, np.int32)
     O = eng.O
     mag = np.zeros((O.N_y, O.N_x), np.float32)
     tot = C.c_double()
     et = xt_abi.PtrTree(E)
+    mp = None
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, np.uint8)
+        if mask.shape != (O.N_time, O.N_y, O.N_x, O.N_z):
+            raise ValueError(f"mask shape {mask.shape}")
+        mp = mask.ctypes.data
     _check(lib().xt_update_lines(*eng.refs(), et.ptr, frame, xs.size, xs.ctypes.data,
-                                 ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot)))
+                                 ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot),
+                                 C.c_void_p(mp)))
     return mag, tot.value
+
+
+def edge_detect(eng: Engine) -> np.ndarray:
+    """The edge-only update mask [N_time][N_y][N_x][N_z] (uint8) of the engine's
+    current object (xt_edge_detect)."""
+    O = eng.O
+    mask = np.zeros((O.N_time, O.N_y, O.N_x, O.N_z), np.uint8)
+    mt = xt_abi.PtrTree(mask)
+    _check(lib().xt_edge_detect(C.byref(O), C.byref(eng.T), mt.ptr))
+    return mask
