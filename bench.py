@@ -453,7 +453,8 @@ def main():
                       "timbir-python_amd/weights/) on the FBP volume, inference timed; "
                       f"multires: the reference launcher's stages ({MULTIRES_ITERS[0]} ICD "
                       f"iterations at 128^2 from zero without offsets, {MULTIRES_ITERS[1]} at "
-                      "256^2, initICD-2 upsampling between them), stage set-up and iterations "
+                      "256^2, initICD-2 upsampling between them; coarse states with the default "
+                      "lines in flight and no FP64 recompute of e), stage set-up and iterations "
                       f"timed; multires_fbp: the same stages ({MULTIRES_FBP_ITERS[0]} and "
                       f"{MULTIRES_FBP_ITERS[1]} iterations) with the 128^2 one started from its "
                       "FBP and ring-offset guess",
