@@ -399,6 +399,11 @@ def main():
                 log(f"{warm} start: within 1% RMSE of the converged image "
                     f"after {c['iters']} iterations, {c['seconds']} s")
             conv = conv if "fbp" in conv else None
+        # the ceiling of the line kernel's e update: its atomic write pattern alone
+        atomic_rate = None
+        if a.workload == "slab":
+            atomic_rate = xt_hip.probe_footprint_atomics()
+            log(f"footprint atomic probe: {atomic_rate:.3e} FP32 atomic lane-ops/s")
         cpu = None
         if not a.no_cpu:
             cpu = cpu_sample(cfg, work, a.cpu_seconds)
@@ -440,7 +445,16 @@ def main():
                                        "/ summed kernel time (HIP events on the library stream)",
                      "kernel": "k_icd_lines", "kernel_ms_per_step": g["kernel_ms"] / a.steps,
                      "launches_per_step": g["launches"] / a.steps,
-                     "bytes_per_step": 12.0 * g["footprint"] / a.steps},
+                     "bytes_per_step": 12.0 * g["footprint"] / a.steps,
+                     "atomic_ceiling": {
+                         "kernel_lane_ops_per_s": (g["footprint"] / (g["kernel_ms"] * 1e-3)
+                                                   if g["kernel_ms"] > 0 else None),
+                         "probe_lane_ops_per_s": atomic_rate,
+                         "frac": (g["footprint"] / (g["kernel_ms"] * 1e-3) / atomic_rate
+                                  if g["kernel_ms"] > 0 and atomic_rate else None),
+                         "basis": "one FP32 atomic per footprint entry of a performed update, "
+                                  "against xt_probe_footprint_atomics (the same write pattern "
+                                  "alone, same run): the line kernel's binding ceiling"}},
         "voxel_updates_per_step": g["updates"] / a.steps,
         "line_visits_per_step": g["visits"] / a.steps,
         "lines_in_flight_per_frame": g["lines_in_flight"],

@@ -273,6 +273,13 @@ int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, i
  * variable).  ICD_BackProject applies it when only_Edge_Updates is set. */
 int xt_edge_detect(ScannedObject *O, TomoInputs *T, uint8_t ****Mask);
 
+/* Measurement probe (no counterpart in the reference): the rate, in FP32 atomic
+ * lane operations per second, of the voxel update's write pattern alone (per view
+ * of a line three bins, sixteen contiguous floats each, no-return buffer atomics;
+ * 1024 views x 768 bins, lines over a 512 x 512 grid) -- the ceiling of the line
+ * kernel's e update on this device.  Allocates and frees its own 50 MB buffer. */
+int xt_probe_footprint_atomics(int32_t n_lines, double *lane_ops_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -236,3 +236,14 @@ def test_autotune_restores_state(scan_root):
     assert res[1][0] == res[0][0]
     assert res[0][1] == 64 and res[1][1] in (16, 32, 64)
     assert res[1][2] < res[1][0]
+
+
+def test_footprint_atomic_probe():
+    """The probe of the line kernel's write pattern runs and reports a rate between
+    1 G and 10 T FP32 atomic lane operations per second."""
+    print("[gpu] footprint atomic probe", flush=True)
+    r = xt_hip.probe_footprint_atomics(2048)
+    print(f"  {r:.3e} lane-ops/s")
+    assert 1e9 < r < 1e13
+    with pytest.raises(xt_hip.XtError):
+        xt_hip.probe_footprint_atomics(0)

@@ -31,7 +31,7 @@ EXPORTS = [
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
-    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines", "xt_edge_detect",
+    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines", "xt_edge_detect", "xt_probe_footprint_atomics",
     "xt_unet_create", "xt_unet_destroy", "xt_unet_param_count", "xt_unet_get_params",
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
@@ -532,6 +532,14 @@ def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int, mask
                                  ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot),
                                  C.c_void_p(mp)))
     return mag, tot.value
+
+
+def probe_footprint_atomics(n_lines: int = 16384) -> float:
+    """FP32 atomic lane operations per second of the voxel update's write pattern
+    alone (xt_probe_footprint_atomics): the ceiling of the line kernel's e update."""
+    r = C.c_double()
+    _check(lib().xt_probe_footprint_atomics(int(n_lines), C.byref(r)))
+    return r.value
 
 
 def edge_detect(eng: Engine) -> np.ndarray:
