@@ -68,19 +68,28 @@ def make_workload(cfg, path, xt_hip, synth):
     return ph
 
 
-def gpu_timed(cfg, path, xt_hip, steps, warmup):
+def gpu_timed(cfg, path, xt_hip, steps, warmup, args=None, volume=None):
+    """steps timed ICD iterations after warmup ones on the stepwise API; volume:
+    the starting object (the structs' own, zero, otherwise)."""
     xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
-    with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
-        icd.iterate(warmup)
-        icd.synchronize()
-        s0 = icd.stats()
-        icd.timing(True)
-        icd.synchronize()
-        t0 = time.perf_counter()
-        icd.iterate(steps)
-        icd.synchronize()
-        t1 = time.perf_counter()
-        s1 = icd.stats()
+    with xt_hip.Engine(path, args or cfg.cli_args()) as eng:
+        if volume is not None:
+            eng.set_volume(volume)
+        with xt_hip.Icd(eng) as icd:
+            return _timed_steps(icd, steps, warmup)
+
+
+def _timed_steps(icd, steps, warmup):
+    icd.iterate(warmup)
+    icd.synchronize()
+    s0 = icd.stats()
+    icd.timing(True)
+    icd.synchronize()
+    t0 = time.perf_counter()
+    icd.iterate(steps)
+    icd.synchronize()
+    t1 = time.perf_counter()
+    s1 = icd.stats()
     dt = t1 - t0
     upd = s1.voxel_updates - s0.voxel_updates
     foot = s1.footprint_entries - s0.footprint_entries
@@ -281,7 +290,7 @@ def bench_unet(a, synth, xt_hip):
                           "tflops": train_tf}}), flush=True)
 
 
-def cpu_sample(cfg, path, target_s):
+def cpu_sample(cfg, path, target_s, phase_contrast=False):
     """Oracle (C restatement, -O3, OpenMP) on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes as C
@@ -292,6 +301,7 @@ def cpu_sample(cfg, path, target_s):
     args = cfg.cli_args()
     args[args.index("--num_threads") + 1] = str(threads)
     o = py_oracle.Oracle(False)
+    o.set_phase_contrast(phase_contrast)
     S, O, T = o.setup(path, args)
     try:
         o.read_sinogram(path, S, T)
@@ -317,6 +327,7 @@ def cpu_sample(cfg, path, target_s):
                     threads=threads, tasks=tasks.value, nb=T.num_z_blocks)
     finally:
         o.free(S, O, T)
+        o.set_phase_contrast(False)
 
 
 def _ratio(conv, slow, fast):
@@ -343,12 +354,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--workdir", default=None)
-    ap.add_argument("--workload", choices=("slab", "4d", "batch", "unet"), default="slab",
+    ap.add_argument("--workload", choices=("slab", "4d", "batch", "unet", "pc", "edge"),
+                    default="slab",
                     help="slab: the 512^2x32 production slab (the metric's workload); 4d: the "
                          "256^2x16 x 16-frame interlaced reconstruction; batch: 24 uneven slabs "
                          "advanced together; unet: the warm-start network's BF16 MFMA kernels "
-                         "(inference on the slab, a training step at 256^2x16) "
-                         "(timing only for 4d, batch and unet)")
+                         "(inference on the slab, a training step at 256^2x16); pc: the "
+                         "slab as a phase-contrast scan (Laplacian footprints); edge: the slab "
+                         "with edge-only updates from the phantom (initICD 1) "
+                         "(timing only for all but slab)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for the roofline traffic figure)")
@@ -373,18 +387,29 @@ def main():
         return bench_batch(a, synth, xt_hip)
     if a.workload == "unet":
         return bench_unet(a, synth, xt_hip)
-    cfg = synth.C2 if a.workload == "slab" else synth.C3
+    cfg = synth.C3 if a.workload == "4d" else synth.C2
     if a.workload != "slab":
         a.no_converge = True  # the converged-image reference exists for the slab only
+    # phase contrast: the scan's projections and the reconstruction use Laplacian footprints
+    xt_hip.set_option(xt_hip.OPT_PHASE_CONTRAST, 1 if a.workload == "pc" else 0)
+    args = cfg.cli_args()
+    if a.workload == "edge":
+        args = cfg.cli_args(init_icd=1) + ["--only_Edge_Updates"]
     work = a.workdir or tempfile.mkdtemp(prefix="xt_bench_")
     try:
         log(f"generating {cfg.name}: {cfg.N_xy}^2 x {cfg.N_z}, {cfg.N_p} views, "
             f"{cfg.N_r}x{cfg.N_t} bins in {work}")
         t0 = time.perf_counter()
-        make_workload(cfg, work, xt_hip, synth)
+        ph = make_workload(cfg, work, xt_hip, synth)
+        if a.workload == "edge":
+            # the starting object the edge mask is taken from: the phantom
+            for f in range(cfg.N_time):
+                ph[f].astype(np.float32).tofile(os.path.join(work, f"object_n0_time_{f}.bin"))
+            np.array([1.0], np.float32).tofile(os.path.join(work, "variance_estimate_n0.bin"))
         log(f"workload ready in {time.perf_counter() - t0:.1f}s; timing {a.steps} iterations "
             f"after {a.warmup} warm-up")
-        g = gpu_timed(cfg, work, xt_hip, a.steps, a.warmup)
+        g = gpu_timed(cfg, work, xt_hip, a.steps, a.warmup, args,
+                      volume=ph if a.workload == "edge" else None)
         log(f"gpu: {g['seconds'] / a.steps * 1e3:.1f} ms/iteration, "
             f"{g['updates'] / g['seconds']:.3e} updates/s")
         conv = None
@@ -406,7 +431,7 @@ def main():
             log(f"footprint atomic probe: {atomic_rate:.3e} FP32 atomic lane-ops/s")
         cpu = None
         if not a.no_cpu:
-            cpu = cpu_sample(cfg, work, a.cpu_seconds)
+            cpu = cpu_sample(cfg, work, a.cpu_seconds, phase_contrast=a.workload == "pc")
             log(f"cpu oracle: {cpu['rate']:.3e} updates/s on {cpu['threads']} threads "
                 f"({cpu['updates']} updates in {cpu['seconds']:.1f}s)")
     finally:
@@ -433,7 +458,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded 12-ellipsoid phantom, Poisson I0=1e4, 0.5% zingers at 5x max, "
                 "ring offsets N(0,0.01)",
-        "config": {"workload": cfg.name, "volume": [cfg.N_xy, cfg.N_xy, cfg.N_z],
+        "config": {"workload": cfg.name + {"pc": "_phase_contrast",
+                                           "edge": "_edge_only"}.get(a.workload, ""),
+                   "volume": [cfg.N_xy, cfg.N_xy, cfg.N_z],
                    "views": cfg.N_p, "bins": [cfg.N_r, cfg.N_t], "step": "one ICD iteration",
                    "resync": "fp64 every iteration", "prior": "qGGMRF p=1.2 q=2 c=0.01",
                    "offset_update": cfg.update_offset},
@@ -443,7 +470,8 @@ def main():
                                      "copy-calibrated; profiles/)",
                      "achieved_basis": "12 B per footprint entry of every performed voxel update "
                                        "/ summed kernel time (HIP events on the library stream)",
-                     "kernel": "k_icd_lines", "kernel_ms_per_step": g["kernel_ms"] / a.steps,
+                     "kernel": "k_icd_lines_pc" if a.workload == "pc" else "k_icd_lines",
+                     "kernel_ms_per_step": g["kernel_ms"] / a.steps,
                      "launches_per_step": g["launches"] / a.steps,
                      "bytes_per_step": 12.0 * g["footprint"] / a.steps,
                      "atomic_ceiling": {

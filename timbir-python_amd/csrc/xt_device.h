@@ -1543,18 +1543,19 @@ __global__ void k_pc_laplace(XtGeom g, const double *__restrict__ ax, int mode,
   }
 }
 
-// Laplacian footprint entry of a column (first candidate bin `first`, values
-// vals[0..XT_MAXA) times delta_t) at bin r and row offset dt from the voxel's
-// row: sum over m = -1, 0, 1 of Lap[m + 1][1 - dt] * line(r + m), in the
-// reference's order and float steps (compute_LapMatrix_4m_AMatrix; zero
-// entries of the line add nothing).
-__device__ __forceinline__ float xt_pc_entry(const XtGeom &g, const float *vals, int first, int r,
+// Laplacian footprint entry of a column (values vals[0..XT_MAXA) times delta_t,
+// vals[k] at bin first + k) at bin first + rr and row offset dt from the voxel's
+// row: sum over m = -1, 0, 1 of Lap[m + 1][1 - dt] * line(first + rr + m), in the
+// reference's order and float steps (compute_LapMatrix_4m_AMatrix; zero entries
+// of the line add nothing).  Called with rr and dt known at compile time (the
+// loops around it unrolled), so vals stays in registers.
+__device__ __forceinline__ float xt_pc_entry(const XtGeom &g, const float (&vals)[XT_MAXA], int rr,
                                              int dt) {
 #pragma clang fp contract(off)
   float acc = 0.f;
 #pragma unroll
   for (int m = -1; m <= 1; m++) {
-    const int k = r + m - first;
+    const int k = rr + m;
     const float lv = (k >= 0 && k < XT_MAXA) ? vals[k] : 0.f;
     acc += g.lap[(m + 1) * 3 + (1 - dt)] * lv;
   }
@@ -1687,12 +1688,15 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
         const float2 a2 = a_v2[p];
         const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
         const int first = a_first[p];
+#pragma unroll
         for (int dt = -1; dt <= 1; dt++) {
           const int t = z + dt;
           if (t < 0 || t >= g.N_t) continue;
-          for (int r = first - 1; r <= first + XT_MAXA; r++) {
+#pragma unroll
+          for (int rr = -1; rr <= XT_MAXA; rr++) {
+            const int r = first + rr;
             if (r < 0 || r >= g.N_r) continue;
-            const float pe = xt_pc_entry(g, vals, first, r, dt);
+            const float pe = xt_pc_entry(g, vals, rr, dt);
             if (pe == 0.f) continue;
             const size_t idx = xt_sidx(g, v, r, t);
             const float e1 = B.e[idx], w1 = B.w[idx];
@@ -1705,54 +1709,60 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
           }
         }
       }
-      red1[tid] = th1;
-      red2[tid] = th2;
+      // workgroup sums: within each wave by shuffles, then the four wave sums
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        th1 += __shfl_xor(th1, o);
+        th2 += __shfl_xor(th2, o);
+      }
+      if ((tid & 63) == 0) {
+        red1[tid >> 6] = th1;
+        red2[tid >> 6] = th2;
+      }
       if (cnt) atomicAdd(&s_cnt, cnt);
       __syncthreads();
-      for (int st = NT / 2; st > 0; st >>= 1) {
-        if (tid < st) {
-          red1[tid] += red1[tid + st];
-          red2[tid] += red2[tid + st];
-        }
-        __syncthreads();
-      }
-      if (tid == 0) {
+      if (tid < 64) {
+        // the qGGMRF terms of the 26 neighbours (lanes 0..26 but 13) and the two
+        // frames (lanes 27, 28), summed over wave 0
         const float V = xcol[z];
         float s1 = 0.f, s2 = 0.f;
-        for (int dz = -1; dz <= 1; dz++)
-          for (int k = 0; k < 9; k++) {
-            const int l = ((dz + 1) * 3 + k / 3) * 3 + k % 3;
-            const int z2 = z + dz, y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
-            if (l == 13 || z2 < 0 || z2 >= g.N_z || y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x)
-              continue;
+        if (tid < 27 && tid != 13) {
+          const int k = tid % 9, z2 = z + tid / 9 - 1;
+          const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+          if (z2 >= 0 && z2 < g.N_z && y2 >= 0 && y2 < g.N_y && x2 >= 0 && x2 < g.N_x) {
             const float nb = xn[k * ZL + z2];
             const float bq =
-                g.filt[l] * xt_qggmrf_ratio(V - nb, g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
-            s1 += bq * nb;
-            s2 += bq;
+                g.filt[tid] * xt_qggmrf_ratio(V - nb, g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+            s1 = bq * nb;
+            s2 = bq;
           }
-        if (f > 0) {
-          const float tn = xn[9 * ZL + z];
+        } else if ((tid == 27 && f > 0) || (tid == 28 && f + 1 < g.N_time)) {
+          const float tn = xn[(tid == 27 ? 9 : 10) * ZL + z];
           const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
-          s1 += bq * tn;
-          s2 += bq;
+          s1 = bq * tn;
+          s2 = bq;
         }
-        if (f + 1 < g.N_time) {
-          const float tn = xn[10 * ZL + z];
-          const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
-          s1 += bq * tn;
-          s2 += bq;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          s1 += __shfl_xor(s1, o);
+          s2 += __shfl_xor(s2, o);
         }
-        const float T1 = red1[0], T2z = red2[0];
-        const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
-        float nv2 = V + g.alpha * (u - V);
-        if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
-        xcol[z] = nv2;
-        s_delta = nv2 - V;
-        magb[zblock(z)] += fabsf(nv2 - V);
-        vox_mag += fabsf(nv2);
-        n_upd += 1.0;
-        n_foot += (double)s_cnt;
+        if (tid == 0) {
+          float T1 = 0.f, T2z = 0.f;
+          for (int w = 0; w < NT / 64; w++) {
+            T1 += red1[w];
+            T2z += red2[w];
+          }
+          const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
+          float nv2 = V + g.alpha * (u - V);
+          if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
+          xcol[z] = nv2;
+          s_delta = nv2 - V;
+          magb[zblock(z)] += fabsf(nv2 - V);
+          vox_mag += fabsf(nv2);
+          n_upd += 1.0;
+          n_foot += (double)s_cnt;
+        }
       }
       __syncthreads();
       const float dl = s_delta;
@@ -1763,12 +1773,15 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
           const float2 a2 = a_v2[p];
           const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
           const int first = a_first[p];
+#pragma unroll
           for (int dt = -1; dt <= 1; dt++) {
             const int t = z + dt;
             if (t < 0 || t >= g.N_t) continue;
-            for (int r = first - 1; r <= first + XT_MAXA; r++) {
+#pragma unroll
+            for (int rr = -1; rr <= XT_MAXA; rr++) {
+              const int r = first + rr;
               if (r < 0 || r >= g.N_r) continue;
-              const float pe = xt_pc_entry(g, vals, first, r, dt);
+              const float pe = xt_pc_entry(g, vals, rr, dt);
               if (pe != 0.f) atomicAdd(&B.e[xt_sidx(g, v, r, t)], -(pe * dl));
             }
           }
