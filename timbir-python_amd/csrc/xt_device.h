@@ -1544,45 +1544,48 @@ __global__ void k_pc_laplace(XtGeom g, const double *__restrict__ ax, int mode,
 }
 
 // Laplacian footprint entry of a column (values vals[0..XT_MAXA) times delta_t,
-// vals[k] at bin first + k) at bin first + rr and row offset dt from the voxel's
-// row: sum over m = -1, 0, 1 of Lap[m + 1][1 - dt] * line(first + rr + m), in the
-// reference's order and float steps (compute_LapMatrix_4m_AMatrix; zero entries
-// of the line add nothing).  Called with rr and dt known at compile time (the
-// loops around it unrolled), so vals stays in registers.
-__device__ __forceinline__ float xt_pc_entry(const XtGeom &g, const float (&vals)[XT_MAXA], int rr,
-                                             int dt) {
+// vals[k] at bin first + k) at bin first + rr, for a row whose Laplacian weights
+// are l0, l1, l2 (Lap[m + 1][1 - dt] for m = -1, 0, 1, dt the row's offset from
+// the voxel's): the sum in the reference's order and float steps
+// (compute_LapMatrix_4m_AMatrix; zero entries of the line add nothing).  rr is
+// known at compile time where it is called, so vals stays in registers.
+__device__ __forceinline__ float xt_pc_entry(const float (&vals)[XT_MAXA], int rr, float l0,
+                                             float l1, float l2) {
 #pragma clang fp contract(off)
+  const float lm = (rr - 1 >= 0 && rr - 1 < XT_MAXA) ? vals[rr - 1] : 0.f;
+  const float lc = (rr >= 0 && rr < XT_MAXA) ? vals[rr] : 0.f;
+  const float lp = (rr + 1 >= 0 && rr + 1 < XT_MAXA) ? vals[rr + 1] : 0.f;
   float acc = 0.f;
-#pragma unroll
-  for (int m = -1; m <= 1; m++) {
-    const int k = rr + m;
-    const float lv = (k >= 0 && k < XT_MAXA) ? vals[k] : 0.f;
-    acc += g.lap[(m + 1) * 3 + (1 - dt)] * lv;
-  }
+  acc += l0 * lm;
+  acc += l1 * lc;
+  acc += l2 * lp;
   return acc;
 }
 
 // Phase-contrast voxel update (F/XT_VoxUpdate.c:590-845).  A slice's footprint
-// covers detector rows t-1..t+1, which its neighbours in the line share, so a
-// line's slices are updated one after another in ascending order (as the
-// reference's slice loop), each with the workgroup's threads spread over the
-// frame's views: theta over the footprint, reduction, the qGGMRF surrogate
-// step on thread 0, e -= A * delta with atomics, and a wait before the next
-// slice reads e.  Lines, the z-block parity and NHICD lists as k_icd_lines.
+// covers detector rows t-1..t+1, so slices three apart have disjoint footprints
+// and no prior term between them: a line's slices are updated in three phases
+// (z mod 3 = 0, 1, 2; or, for the per-voxel checks, one slice per phase in
+// ascending order, the reference's slice loop), the slices of a phase at once.
+// Lanes run over detector rows: lane t of a view group serves the phase's slice
+// among t-1..t+1, so a wave reads and updates whole rows of e.  Theta is summed
+// over the view groups in LDS and over the slice's three rows by the slice's own
+// thread, which takes the qGGMRF step; then e -= A * delta and a wait before the
+// next phase reads e.  Lines, the z-block parity and NHICD lists as k_icd_lines.
 template <int ZL>
 __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
-  constexpr int NT = 256;
+  constexpr int NT = 256, RL = ZL, G = NT / RL;  // row lanes of a view group, view groups
   extern __shared__ float4 smem4[];
-  float4 *a_v4 = smem4;                          // [max_views] bins 0..3 (x delta_t)
+  float4 *a_v4 = smem4;                           // [max_views] bins 0..3 (x delta_t)
   float2 *a_v2 = (float2 *)(a_v4 + g.max_views);  // [max_views] bins 4..5
   int *a_first = (int *)(a_v2 + g.max_views);     // [max_views] first candidate bin
-  float *red1 = (float *)(a_first + g.max_views);  // [NT]
-  float *red2 = red1 + NT;                        // [NT]
-  float *xn = red2 + NT;                          // [11][ZL]
-  float *magb = xn + 11 * ZL;                     // [64]
-  __shared__ float s_delta;
-  __shared__ int s_skip, s_cnt;
-  const int tid = threadIdx.x;
+  float *red1 = (float *)(a_first + g.max_views);  // [G][RL]
+  float *red2 = red1 + NT;                         // [G][RL]
+  float *xn = red2 + NT;                           // [11][ZL]
+  float *delta = xn + 11 * ZL;                     // [ZL]
+  float *magb = delta + ZL;                        // [64]
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x, grp = tid / RL, lt = tid % RL;
   const int f = blockIdx.y;
   const int nv = B.proj_num[f];
   const int *vlist = B.views + (size_t)f * g.max_views;
@@ -1617,11 +1620,22 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
     istep = 1;
   }
   const float T2 = g.T * g.T, Td = g.T * g.delta;
+  // the Laplacian weights of the lane's row for each row offset dt = -1, 0, 1
+  // come from column 1 - dt of Lap_Kernel
+  const int nph = L.zorder ? (g.N_z > 3 ? g.N_z : 3) : 3;
   float vox_mag = 0.f;
   double n_upd = 0.0, n_foot = 0.0, n_vis = 0.0;
   for (int i = i0; i < i1; i += istep) {
     const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
+    // a slice is updated when it is in the launch's parity, its block selected
+    // the line (NHICD) and the edge mask (if any) holds it
+    auto slice_on = [&](int z) {
+      bool upd = par < 0 || (zblock(z) & 1) == par;
+      if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+      if (upd && B.vmask) upd = B.vmask[((size_t)f * NYX + c) * g.N_z + z] != 0;
+      return upd;
+    };
     for (int p = tid; p < nv; p += NT) {
       const int v = vlist[p];
       int first = 0;
@@ -1652,53 +1666,43 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
       xn[q] = val;
     }
     if (tid < 64) magb[tid] = 0.f;
+    if (tid < ZL) delta[tid] = 0.f;
+    if (tid == 0) s_cnt = 0;
     __syncthreads();
     float *xcol = xn + 4 * ZL;
-    for (int z = 0; z < g.N_z; z++) {
-      bool upd = par < 0 || (zblock(z) & 1) == par;
-      if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
-      if (upd && B.vmask) upd = B.vmask[((size_t)f * NYX + c) * g.N_z + z] != 0;
-      if (!upd) continue;  // uniform over the workgroup
-      // zero skipping, F/XT_VoxUpdate.c:805-826 (the 26 neighbours and two frames)
-      if (tid == 0) {
-        const float V = xcol[z];
-        bool any_pos = false;
-        for (int dz = -1; dz <= 1; dz++)
-          for (int k = 0; k < 9; k++) {
-            const int z2 = z + dz;
-            if ((dz == 0 && k == 4) || z2 < 0 || z2 >= g.N_z) continue;
-            const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
-            if (y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x) continue;
-            any_pos |= xn[k * ZL + z2] > 0.f;
-          }
-        any_pos |= (f > 0 && xn[9 * ZL + z] > 0.f) || (f + 1 < g.N_time && xn[10 * ZL + z] > 0.f);
-        s_skip = (V == 0.f) && (iter > 1) && !any_pos;
-        s_cnt = 0;
+    for (int ph = 0; ph < nph; ph++) {
+      // the lane's slice in this phase: the one of rows lt-1..lt+1 whose index
+      // is ph modulo the phase count (at most one), and the row's offset dt
+      int zt = -1, dt = 0;
+#pragma unroll
+      for (int q = -1; q <= 1; q++) {
+        const int z = lt + q;
+        if (z >= 0 && z < g.N_z && z % nph == ph) {
+          zt = z;
+          dt = -q;
+        }
       }
-      __syncthreads();
-      const int skip = s_skip;
-      __syncthreads();  // every thread has read s_skip before the next slice rewrites it
-      if (skip) continue;
-      // theta over the footprint rows z-1..z+1, bins first-1..first+XT_MAXA
+      const bool act = zt >= 0 && lt < g.N_t && slice_on(zt);
+      const int col = 1 - dt;
+      const float l0 = col == 0 ? g.lap[0] : (col == 1 ? g.lap[1] : g.lap[2]);
+      const float l1 = col == 0 ? g.lap[3] : (col == 1 ? g.lap[4] : g.lap[5]);
+      const float l2 = col == 0 ? g.lap[6] : (col == 1 ? g.lap[7] : g.lap[8]);
       float th1 = 0.f, th2 = 0.f;
       int cnt = 0;
-      for (int p = tid; p < nv; p += NT) {
-        const int v = vlist[p];
-        const float4 a4 = a_v4[p];
-        const float2 a2 = a_v2[p];
-        const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
-        const int first = a_first[p];
-#pragma unroll
-        for (int dt = -1; dt <= 1; dt++) {
-          const int t = z + dt;
-          if (t < 0 || t >= g.N_t) continue;
+      if (act) {
+        for (int p = grp; p < nv; p += G) {
+          const int v = vlist[p];
+          const float4 a4 = a_v4[p];
+          const float2 a2 = a_v2[p];
+          const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
+          const int first = a_first[p];
 #pragma unroll
           for (int rr = -1; rr <= XT_MAXA; rr++) {
             const int r = first + rr;
             if (r < 0 || r >= g.N_r) continue;
-            const float pe = xt_pc_entry(g, vals, rr, dt);
+            const float pe = xt_pc_entry(vals, rr, l0, l1, l2);
             if (pe == 0.f) continue;
-            const size_t idx = xt_sidx(g, v, r, t);
+            const size_t idx = xt_sidx(g, v, r, lt);
             const float e1 = B.e[idx], w1 = B.w[idx];
             const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
             const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
@@ -1709,85 +1713,84 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
           }
         }
       }
-      // workgroup sums: within each wave by shuffles, then the four wave sums
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        th1 += __shfl_xor(th1, o);
-        th2 += __shfl_xor(th2, o);
-      }
-      if ((tid & 63) == 0) {
-        red1[tid >> 6] = th1;
-        red2[tid >> 6] = th2;
-      }
+      red1[tid] = th1;
+      red2[tid] = th2;
       if (cnt) atomicAdd(&s_cnt, cnt);
       __syncthreads();
-      if (tid < 64) {
-        // the qGGMRF terms of the 26 neighbours (lanes 0..26 but 13) and the two
-        // frames (lanes 27, 28), summed over wave 0
+      // the phase's slices, one thread each: theta over its three rows and the
+      // view groups, zero skipping (F/XT_VoxUpdate.c:805-826), the qGGMRF step
+      if (tid < g.N_z && tid % nph == ph && slice_on(tid)) {
+        const int z = tid;
+        float T1 = 0.f, T2z = 0.f;
+        for (int q = -1; q <= 1; q++) {
+          const int t = z + q;
+          if (t < 0 || t >= g.N_t) continue;
+          for (int gg = 0; gg < G; gg++) {
+            T1 += red1[gg * RL + t];
+            T2z += red2[gg * RL + t];
+          }
+        }
         const float V = xcol[z];
+        bool any_pos = false;
         float s1 = 0.f, s2 = 0.f;
-        if (tid < 27 && tid != 13) {
-          const int k = tid % 9, z2 = z + tid / 9 - 1;
-          const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
-          if (z2 >= 0 && z2 < g.N_z && y2 >= 0 && y2 < g.N_y && x2 >= 0 && x2 < g.N_x) {
+        for (int dz = -1; dz <= 1; dz++)
+          for (int k = 0; k < 9; k++) {
+            const int l = (dz + 1) * 9 + k;
+            const int z2 = z + dz, y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+            if (l == 13 || z2 < 0 || z2 >= g.N_z || y2 < 0 || y2 >= g.N_y || x2 < 0 || x2 >= g.N_x)
+              continue;
             const float nb = xn[k * ZL + z2];
+            any_pos |= nb > 0.f;
             const float bq =
-                g.filt[tid] * xt_qggmrf_ratio(V - nb, g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
-            s1 = bq * nb;
-            s2 = bq;
+                g.filt[l] * xt_qggmrf_ratio(V - nb, g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+            s1 += bq * nb;
+            s2 += bq;
           }
-        } else if ((tid == 27 && f > 0) || (tid == 28 && f + 1 < g.N_time)) {
-          const float tn = xn[(tid == 27 ? 9 : 10) * ZL + z];
+        if (f > 0) {
+          const float tn = xn[9 * ZL + z];
+          any_pos |= tn > 0.f;
           const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
-          s1 = bq * tn;
-          s2 = bq;
+          s1 += bq * tn;
+          s2 += bq;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          s1 += __shfl_xor(s1, o);
-          s2 += __shfl_xor(s2, o);
+        if (f + 1 < g.N_time) {
+          const float tn = xn[10 * ZL + z];
+          any_pos |= tn > 0.f;
+          const float bq = g.tfilt * xt_qggmrf_ratio(V - tn, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+          s1 += bq * tn;
+          s2 += bq;
         }
-        if (tid == 0) {
-          float T1 = 0.f, T2z = 0.f;
-          for (int w = 0; w < NT / 64; w++) {
-            T1 += red1[w];
-            T2z += red2[w];
-          }
+        if (!((V == 0.f) && (iter > 1) && !any_pos)) {
           const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
           float nv2 = V + g.alpha * (u - V);
           if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
           xcol[z] = nv2;
-          s_delta = nv2 - V;
-          magb[zblock(z)] += fabsf(nv2 - V);
+          delta[z] = nv2 - V;
+          atomicAdd(&magb[zblock(z)], fabsf(nv2 - V));
           vox_mag += fabsf(nv2);
           n_upd += 1.0;
-          n_foot += (double)s_cnt;
         }
       }
       __syncthreads();
-      const float dl = s_delta;
+      // e -= A * delta over the lane's row
+      const float dl = act ? delta[zt] : 0.f;
       if (dl != 0.f) {
-        for (int p = tid; p < nv; p += NT) {
+        for (int p = grp; p < nv; p += G) {
           const int v = vlist[p];
           const float4 a4 = a_v4[p];
           const float2 a2 = a_v2[p];
           const float vals[XT_MAXA] = {a4.x, a4.y, a4.z, a4.w, a2.x, a2.y};
           const int first = a_first[p];
 #pragma unroll
-          for (int dt = -1; dt <= 1; dt++) {
-            const int t = z + dt;
-            if (t < 0 || t >= g.N_t) continue;
-#pragma unroll
-            for (int rr = -1; rr <= XT_MAXA; rr++) {
-              const int r = first + rr;
-              if (r < 0 || r >= g.N_r) continue;
-              const float pe = xt_pc_entry(g, vals, rr, dt);
-              if (pe != 0.f) atomicAdd(&B.e[xt_sidx(g, v, r, t)], -(pe * dl));
-            }
+          for (int rr = -1; rr <= XT_MAXA; rr++) {
+            const int r = first + rr;
+            if (r < 0 || r >= g.N_r) continue;
+            const float pe = xt_pc_entry(vals, rr, l0, l1, l2);
+            if (pe != 0.f) atomicAdd(&B.e[xt_sidx(g, v, r, lt)], -(pe * dl));
           }
         }
       }
-      // the next slice's theta reads these e entries
+      // the next phase's theta reads these e entries
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1809,16 +1812,15 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
       for (int b = 0; b < g.nb; b++) sm += magb[b];
       B.magsum[(size_t)f * NYX + c] = sm;
       n_vis += 1.0;
+      n_foot += (double)s_cnt;
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    if (L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
-    if (n_vis != 0.0) {
-      atomicAdd(&B.stats[1], n_upd);
-      atomicAdd(&B.stats[2], n_foot);
-      atomicAdd(&B.stats[3], n_vis);
-    }
+  if (L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
+  if (n_upd != 0.0) atomicAdd(&B.stats[1], n_upd);
+  if (tid == 0 && n_vis != 0.0) {
+    atomicAdd(&B.stats[2], n_foot);
+    atomicAdd(&B.stats[3], n_vis);
   }
 }
 
