@@ -1326,62 +1326,80 @@ __global__ void k_vsc_filter(int NY, int NX, const float *__restrict__ mag, floa
   filt[(size_t)m * NYX + idx] = acc;
 }
 
-__global__ void __launch_bounds__(1024) k_vsc_select(int NYX, const float *__restrict__ filt,
-                                                     const int *__restrict__ n_select,
-                                                     uint8_t *__restrict__ flag,
-                                                     int *__restrict__ n_out) {
-  __shared__ unsigned int hist[256];
-  __shared__ unsigned int s_prefix, s_mask;
-  __shared__ int s_target;
+// NHICD threshold per map m (F/XT_NHICD.c:178-205): the order statistic of rank
+// NYX - n_select (1-indexed from the smallest) of the filtered map, found by a
+// radix selection on the float bits (non-negative floats order as unsigned
+// integers), 8 bits a pass; lines strictly above it are selected.  The passes run
+// over many workgroups: state st[m] = {prefix, mask, rank left}, histogram
+// hist[m][256] (cleared by k_vsc_init and by each pick, so a captured graph can
+// replay them).
+__global__ void k_vsc_init(int NYX, const int *__restrict__ n_select, unsigned *__restrict__ st,
+                           unsigned *__restrict__ hist, int *__restrict__ n_out) {
   const int m = blockIdx.x;
-  const float *F = filt + (size_t)m * NYX;
-  const unsigned int *K = (const unsigned int *)F;
-  // 1-indexed rank from the smallest: i = N - n_select (F/XT_NHICD.c:185)
-  int target = NYX - n_select[m] - 1;  // 0-indexed
-  if (target < 0) target = 0;
+  hist[(size_t)m * 256 + threadIdx.x] = 0u;
   if (threadIdx.x == 0) {
-    s_prefix = 0;
-    s_mask = 0;
-    s_target = target;
+    int target = NYX - n_select[m] - 1;  // 0-indexed
+    if (target < 0) target = 0;
+    st[m * 4 + 0] = 0u;
+    st[m * 4 + 1] = 0u;
+    st[m * 4 + 2] = (unsigned)target;
+    n_out[m] = 0;
+  }
+}
+
+__global__ void k_vsc_hist(int NYX, const float *__restrict__ filt, const unsigned *__restrict__ st,
+                           unsigned *__restrict__ hist, int shift) {
+  __shared__ unsigned h[256];
+  const int m = blockIdx.y;
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const unsigned pre = st[m * 4 + 0], msk = st[m * 4 + 1];
+  const unsigned *K = (const unsigned *)(filt + (size_t)m * NYX);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NYX; i += gridDim.x * blockDim.x) {
+    const unsigned k = K[i];
+    if ((k & msk) == pre) atomicAdd(&h[(k >> shift) & 255u], 1u);
   }
   __syncthreads();
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
-    __syncthreads();
-    const unsigned int pre = s_prefix, msk = s_mask;
-    for (int i = threadIdx.x; i < NYX; i += blockDim.x) {
-      const unsigned int k = K[i];
-      if ((k & msk) == pre) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+  if (h[threadIdx.x]) atomicAdd(&hist[(size_t)m * 256 + threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_vsc_pick(unsigned *__restrict__ st, unsigned *__restrict__ hist, int shift) {
+  const int m = blockIdx.x;
+  unsigned *H = hist + (size_t)m * 256;
+  if (threadIdx.x == 0) {
+    const unsigned t = st[m * 4 + 2];
+    unsigned cum = 0u;
+    int b = 0;
+    for (; b < 256; b++) {
+      if (cum + H[b] > t) break;
+      cum += H[b];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int t = s_target;
-      unsigned int cum = 0;
-      int b = 0;
-      for (; b < 256; b++) {
-        if (cum + hist[b] > (unsigned int)t) break;
-        cum += hist[b];
-      }
-      if (b > 255) b = 255;
-      s_target = t - (int)cum;
-      s_prefix = pre | ((unsigned int)b << shift);
-      s_mask = msk | (255u << shift);
-    }
-    __syncthreads();
+    if (b > 255) b = 255;
+    st[m * 4 + 2] = t - cum;
+    st[m * 4 + 0] |= (unsigned)b << shift;
+    st[m * 4 + 1] |= 255u << shift;
   }
-  const float thresh = __uint_as_float(s_prefix);
+  __syncthreads();
+  H[threadIdx.x] = 0u;
+}
+
+__global__ void k_vsc_flag(int NYX, const float *__restrict__ filt, const unsigned *__restrict__ st,
+                           uint8_t *__restrict__ flag, int *__restrict__ n_out) {
   __shared__ int cnt;
+  const int m = blockIdx.y;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
+  const float thresh = __uint_as_float(st[m * 4 + 0]);
+  const float *F = filt + (size_t)m * NYX;
   int local = 0;
-  for (int i = threadIdx.x; i < NYX; i += blockDim.x) {
-    const uint8_t s = F[i] > thresh ? 1 : 0;
-    flag[(size_t)m * NYX + i] = s;
-    local += s;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NYX; i += gridDim.x * blockDim.x) {
+    const uint8_t sel = F[i] > thresh ? 1 : 0;
+    flag[(size_t)m * NYX + i] = sel;
+    local += sel;
   }
-  atomicAdd(&cnt, local);
+  if (local) atomicAdd(&cnt, local);
   __syncthreads();
-  if (threadIdx.x == 0) n_out[m] = cnt;
+  if (threadIdx.x == 0 && cnt) atomicAdd(&n_out[m], cnt);
 }
 
 // A line is an NHICD line of a z-block parity when a z-block of that parity

@@ -242,6 +242,7 @@ struct xt_icd_state {
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
   float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr, *magpar = nullptr;
   int *nsel = nullptr, *nsel_out = nullptr;
+  unsigned *vsc_st = nullptr, *vsc_hist = nullptr;  // NHICD radix selection scratch
   double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
   double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
   float4 *at_vals = nullptr;  // A-column table (null: columns built on the fly)
@@ -274,7 +275,8 @@ static void free_state(xt_icd_state *s) {
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
-                  s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp, s->vmask};
+                  s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp, s->vmask, s->vsc_st,
+                  s->vsc_hist};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *b : bufs)
@@ -480,6 +482,19 @@ static int variance_update(xt_icd_state *s) {
   return XT_OK;
 }
 
+// The NHICD threshold and flags of M filtered maps (k_vsc_init, four radix passes,
+// k_vsc_flag); st holds 4 and hist 256 words per map.
+static void launch_vsc_select(hipStream_t st, int NYX, int M, const float *filt, const int *nsel,
+                              uint8_t *flag, int *nout, unsigned *vst, unsigned *hist) {
+  const dim3 grid((unsigned)std::max(1, std::min(64, (NYX + 4095) / 4096)), M);
+  hipLaunchKernelGGL(k_vsc_init, dim3(M), dim3(256), 0, st, NYX, nsel, vst, hist, nout);
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_vsc_hist, grid, dim3(256), 0, st, NYX, filt, vst, hist, shift);
+    hipLaunchKernelGGL(k_vsc_pick, dim3(M), dim3(256), 0, st, vst, hist, shift);
+  }
+  hipLaunchKernelGGL(k_vsc_flag, grid, dim3(256), 0, st, NYX, filt, vst, flag, nout);
+}
+
 // NHICD selection per (frame, z-block) map as F/XT_NHICD.c:178-205, then the
 // per-line union that decides which lines the NHICD launches visit.
 static int vsc_select(xt_icd_state *s) {
@@ -493,8 +508,8 @@ static int vsc_select(xt_icd_state *s) {
     dim3 gf((unsigned)((s->NYX + 255) / 256), M);
     hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->magpar,
                        s->filt);
-    hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
-                       s->nhicd_flag, s->nsel_out);
+    launch_vsc_select(s->st, (int)s->NYX, M, s->filt, s->nsel, s->nhicd_flag, s->nsel_out,
+                      s->vsc_st, s->vsc_hist);
     hipLaunchKernelGGL(k_flag_blocks, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX,
                        s->g.N_time, s->nhicd_flag, s->nhicd_blk);
   } else {
@@ -502,8 +517,8 @@ static int vsc_select(xt_icd_state *s) {
     dim3 gf((unsigned)((s->NYX + 255) / 256), M);
     hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->mag_blk,
                        s->filt);
-    hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, s->st, (int)s->NYX, s->filt, s->nsel,
-                       s->nhicd_blk, s->nsel_out);
+    launch_vsc_select(s->st, (int)s->NYX, M, s->filt, s->nsel, s->nhicd_blk, s->nsel_out,
+                      s->vsc_st, s->vsc_hist);
     hipLaunchKernelGGL(k_vsc_any, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX, s->g.N_time,
                        s->nhicd_blk, s->nhicd_flag);
   }
@@ -969,6 +984,8 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       (rc = dev_alloc(&s->magpar, (size_t)g.N_time * 2 * s->NYX)) ||
       (rc = dev_alloc(&s->nsel, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
       (rc = dev_alloc(&s->nsel_out, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
+      (rc = dev_alloc(&s->vsc_st, (size_t)4 * g.N_time * (g.nb > 2 ? g.nb : 2))) ||
+      (rc = dev_alloc(&s->vsc_hist, (size_t)256 * g.N_time * (g.nb > 2 ? g.nb : 2))) ||
       (rc = dev_alloc(&s->stats, 8)) || (rc = dev_alloc(&s->red_wg, 8192)) ||
       (rc = dev_alloc(&s->scal, 8)) || (rc = dev_alloc(&s->iter_dev, 1)))
     return fail(rc);
@@ -2088,10 +2105,12 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
   float *mag = nullptr, *filt = nullptr;
   int *nsel = nullptr, *nout = nullptr;
   uint8_t *flag = nullptr;
+  unsigned *vst = nullptr, *vhist = nullptr;
   int rc = XT_OK;
   std::vector<uint8_t> hflag((size_t)M * NYX);
   if ((rc = dev_alloc(&mag, M * NYX)) || (rc = dev_alloc(&filt, M * NYX)) ||
-      (rc = dev_alloc(&nsel, M)) || (rc = dev_alloc(&nout, M)) || (rc = dev_alloc(&flag, M * NYX))) {
+      (rc = dev_alloc(&nsel, M)) || (rc = dev_alloc(&nout, M)) || (rc = dev_alloc(&flag, M * NYX)) ||
+      (rc = dev_alloc(&vst, (size_t)4 * M)) || (rc = dev_alloc(&vhist, (size_t)256 * M))) {
   } else {
     hipError_t e = hipMemcpy(mag, &MagUpdateMap[0][0][0][0], M * NYX * sizeof(float),
                              hipMemcpyHostToDevice);
@@ -2099,13 +2118,13 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((NYX + 255) / 256), M), dim3(256), 0, 0, NY,
                          NX, mag, filt);
-      hipLaunchKernelGGL(k_vsc_select, dim3(M), dim3(1024), 0, 0, (int)NYX, filt, nsel, flag, nout);
+      launch_vsc_select(0, (int)NYX, M, filt, nsel, flag, nout, vst, vhist);
       e = hipDeviceSynchronize();
     }
     if (e == hipSuccess) e = hipMemcpy(hflag.data(), flag, M * NYX, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = set_err(XT_E_HIP, "vsc seam: %s", hipGetErrorString(e));
   }
-  void *b[] = {mag, filt, nsel, nout, flag};
+  void *b[] = {mag, filt, nsel, nout, flag, vst, vhist};
   for (void *p : b)
     if (p) (void)hipFree(p);
   if (rc) return rc;
