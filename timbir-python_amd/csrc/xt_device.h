@@ -1038,38 +1038,65 @@ __global__ void k_flag_blocks(int nb, size_t NYX, int n_time, const uint8_t *__r
 }
 
 // NHICD line list of each (frame, parity) map: the selected lines inside the
-// mask, in raster order, and their number.  One workgroup of 1024 threads per
-// map.
-__global__ void __launch_bounds__(1024) k_nhicd_compact(int NYX, const uint8_t *__restrict__ flag,
-                                                        const uint8_t *__restrict__ mask,
-                                                        int *__restrict__ list,
-                                                        int *__restrict__ count) {
-  __shared__ int wsum[16];
-  __shared__ int s_base;
-  const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+// mask, in raster order, and their number.  The map is cut into chunks of
+// XT_COMPACT_CHUNK lines, one 256-thread workgroup each (grid = (chunks, maps)):
+// k_nhicd_chunk_count counts every chunk's lines into cnt[map][chunk], then
+// k_nhicd_chunk_write places a chunk's lines after those of the earlier chunks
+// and its last chunk stores the map's total.
+#define XT_COMPACT_CHUNK 4096
+
+__device__ __forceinline__ int xt_block256_sum(int v, int *wsum) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ void __launch_bounds__(256) k_nhicd_chunk_count(int NYX, int nch,
+                                                           const uint8_t *__restrict__ flag,
+                                                           const uint8_t *__restrict__ mask,
+                                                           int *__restrict__ cnt) {
+  __shared__ int wsum[4];
+  const int ch = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+  const uint8_t *fl = flag + (size_t)f * NYX;
+  const int c0 = ch * XT_COMPACT_CHUNK;
+  int n = 0;
+  for (int k = 0; k < XT_COMPACT_CHUNK / 256; k++) {
+    const int c = c0 + k * 256 + tid;
+    n += (c < NYX && fl[c] && mask[c]) ? 1 : 0;
+  }
+  n = xt_block256_sum(n, wsum);
+  if (tid == 0) cnt[(size_t)f * nch + ch] = n;
+}
+
+__global__ void __launch_bounds__(256) k_nhicd_chunk_write(int NYX, int nch,
+                                                           const uint8_t *__restrict__ flag,
+                                                           const uint8_t *__restrict__ mask,
+                                                           const int *__restrict__ cnt,
+                                                           int *__restrict__ list,
+                                                           int *__restrict__ count) {
+  __shared__ int wsum[4];
+  const int ch = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int *cf = cnt + (size_t)f * nch;
+  int b = 0;
+  for (int j = tid; j < ch; j += 256) b += cf[j];
+  int base = xt_block256_sum(b, wsum);
   const uint8_t *fl = flag + (size_t)f * NYX;
   int *out = list + (size_t)f * NYX;
-  if (tid == 0) s_base = 0;
-  __syncthreads();
-  for (int c0 = 0; c0 < NYX; c0 += 1024) {
-    const int c = c0 + tid;
+  const int c0 = ch * XT_COMPACT_CHUNK;
+  for (int k = 0; k < XT_COMPACT_CHUNK / 256; k++) {
+    const int c = c0 + k * 256 + tid;
     const bool on = c < NYX && fl[c] && mask[c];
     const unsigned long long bal = __ballot(on);
-    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    __syncthreads();  // every wave has read wsum of the previous step
     if (lane == 0) wsum[wid] = __popcll(bal);
     __syncthreads();
-    int off = s_base;
-    for (int k = 0; k < wid; k++) off += wsum[k];
-    if (on) out[off + before] = c;
-    __syncthreads();
-    if (tid == 0) {
-      int t = 0;
-      for (int k = 0; k < 16; k++) t += wsum[k];
-      s_base += t;
-    }
-    __syncthreads();
+    int off = base;
+    for (int q = 0; q < wid; q++) off += wsum[q];
+    if (on) out[off + __popcll(bal & ((1ull << lane) - 1ull))] = c;
+    base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
-  if (tid == 0) count[f] = s_base;
+  if (ch == nch - 1 && tid == 0) count[f] = base;
 }
 
 // ---------------------------------------------------------------------------

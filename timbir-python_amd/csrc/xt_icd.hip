@@ -238,6 +238,7 @@ struct xt_icd_state {
   float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr, *fbp_w = nullptr;
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
   int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
+  int *nhicd_chunk = nullptr;  // NHICD list compaction: lines per chunk of each map
   uint8_t *line_mask = nullptr;
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
   float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr, *magpar = nullptr;
@@ -270,7 +271,7 @@ static void free_state(xt_icd_state *s) {
   void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
                   s->fbp_w,
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
-                  s->lines,    s->nhicd_list, s->nhicd_count, s->line_mask, s->nhicd_flag,
+                  s->lines,    s->nhicd_list, s->nhicd_count, s->nhicd_chunk, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
                   s->magpar,
@@ -522,8 +523,12 @@ static int vsc_select(xt_icd_state *s) {
     hipLaunchKernelGGL(k_vsc_any, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX, s->g.N_time,
                        s->nhicd_blk, s->nhicd_flag);
   }
-  hipLaunchKernelGGL(k_nhicd_compact, dim3(s->g.N_time * 2), dim3(1024), 0, s->st, (int)s->NYX,
-                     s->nhicd_flag, s->line_mask, s->nhicd_list, s->nhicd_count);
+  const int nch = (int)((s->NYX + XT_COMPACT_CHUNK - 1) / XT_COMPACT_CHUNK);
+  const dim3 gc((unsigned)nch, (unsigned)(s->g.N_time * 2));
+  hipLaunchKernelGGL(k_nhicd_chunk_count, gc, dim3(256), 0, s->st, (int)s->NYX, nch,
+                     s->nhicd_flag, s->line_mask, s->nhicd_chunk);
+  hipLaunchKernelGGL(k_nhicd_chunk_write, gc, dim3(256), 0, s->st, (int)s->NYX, nch,
+                     s->nhicd_flag, s->line_mask, s->nhicd_chunk, s->nhicd_list, s->nhicd_count);
   XT_HIP(hipGetLastError());
   return XT_OK;
 }
@@ -548,6 +553,8 @@ static int setup_lines(xt_icd_state *s, const std::vector<int> &masked) {
   XT_TRY(dev_alloc(&s->line_mask, s->NYX));
   XT_TRY(dev_alloc(&s->nhicd_list, (size_t)s->g.N_time * 2 * s->NYX));
   XT_TRY(dev_alloc(&s->nhicd_count, s->g.N_time * 2));
+  XT_TRY(dev_alloc(&s->nhicd_chunk, (size_t)s->g.N_time * 2 *
+                                        ((s->NYX + XT_COMPACT_CHUNK - 1) / XT_COMPACT_CHUNK)));
   if (!masked.empty())
     XT_HIP(hipMemcpy(s->lines, masked.data(), masked.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->line_mask, in.data(), s->NYX, hipMemcpyHostToDevice));
