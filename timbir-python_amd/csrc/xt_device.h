@@ -1104,6 +1104,21 @@ __global__ void __launch_bounds__(256) k_nhicd_chunk_write(int NYX, int nch,
 // detector pixel is split over view chunks (enough workgroups to fill the
 // chip), then combined, then the zero-mean constraint gamma is formed, then
 // the correction is applied to d and to e.  Sums are FP64.
+// One sinogram entry's terms of the offset step's numerator and denominator,
+// F/XT_ICD_update.c:703-717 (ProjSelect is the threshold test).
+__device__ __forceinline__ void offset_accumulate(const XtGeom &g, float ev, float wv, double &num,
+                                                  double &den) {
+  if (ev * ev * wv < g.T * g.T) {
+    num += (double)ev * (double)wv;
+    den += (double)wv;
+  } else {
+    const double temp = (double)g.T * (double)g.delta * sqrt((double)wv);
+    const double sign = (ev > 0.f) - (ev < 0.f);
+    num += temp * sign;
+    den += temp / fabs((double)ev);
+  }
+}
+
 __global__ void k_offset_partial(XtGeom g, const float *__restrict__ e, const float *__restrict__ w,
                                  int chunk_views, double *__restrict__ part /*[nch][2][NRT]*/) {
   const int NRT = g.N_r * g.N_t;
@@ -1113,18 +1128,23 @@ __global__ void k_offset_partial(XtGeom g, const float *__restrict__ e, const fl
   const int p0 = ch * chunk_views, p1 = min(g.N_p, p0 + chunk_views);
   double num = 0.0, den = 0.0;
   const int r = rt / g.N_t, t = rt - (rt / g.N_t) * g.N_t;
-  for (int p = p0; p < p1; p++) {
-    const size_t idx = xt_sidx(g, p, r, t);
-    const float ev = e[idx], wv = w[idx];
-    if (ev * ev * wv < g.T * g.T) {
-      num += (double)ev * (double)wv;
-      den += (double)wv;
-    } else {
-      const double temp = (double)g.T * (double)g.delta * sqrt((double)wv);
-      const double sign = (ev > 0.f) - (ev < 0.f);
-      num += temp * sign;
-      den += temp / fabs((double)ev);
+  // views in groups of 8: the group's loads are issued before its sums, which
+  // keep the view order
+  int p = p0;
+  for (; p + 8 <= p1; p += 8) {
+    float ev[8], wv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const size_t idx = xt_sidx(g, p + k, r, t);
+      ev[k] = e[idx];
+      wv[k] = w[idx];
     }
+#pragma unroll
+    for (int k = 0; k < 8; k++) offset_accumulate(g, ev[k], wv[k], num, den);
+  }
+  for (; p < p1; p++) {
+    const size_t idx = xt_sidx(g, p, r, t);
+    offset_accumulate(g, e[idx], w[idx], num, den);
   }
   part[((size_t)ch * 2 + 0) * NRT + rt] = num;
   part[((size_t)ch * 2 + 1) * NRT + rt] = den;
