@@ -217,8 +217,12 @@ def test_variance_update(scan_root, oracle64a):
         oracle64a.free(S, O, T)
 
 
-def test_vsc_select_bitexact(scan_root, oracle32):
-    cfg = SMALL4D
+# 96x96 maps: 9216 lines, three chunks of the line-list compaction (the last partial)
+VSC96 = dataclasses.replace(TINY, name="vsc96", N_xy=96, N_p=48, seed=17)
+
+
+@pytest.mark.parametrize("cfg", [SMALL4D, VSC96], ids=lambda c: c.name)
+def test_vsc_select_bitexact(cfg, scan_root, oracle32):
     _print_case("vsc_select", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     try:

@@ -2109,15 +2109,18 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
   for (int i = 0; i < O->N_time; i++)
     for (int b = 0; b < nb; b++)
       ns[i * nb + b] = (int)floor((double)(T->UpdateSelectNum[i][b] / O->NHICD_Iterations));
+  const int nch = (int)((NYX + XT_COMPACT_CHUNK - 1) / XT_COMPACT_CHUNK);
   float *mag = nullptr, *filt = nullptr;
-  int *nsel = nullptr, *nout = nullptr;
-  uint8_t *flag = nullptr;
+  int *nsel = nullptr, *nout = nullptr, *cnt = nullptr, *list = nullptr, *count = nullptr;
+  uint8_t *flag = nullptr, *all = nullptr;
   unsigned *vst = nullptr, *vhist = nullptr;
   int rc = XT_OK;
-  std::vector<uint8_t> hflag((size_t)M * NYX);
+  std::vector<int> hlist((size_t)M * NYX), hcount(M);
   if ((rc = dev_alloc(&mag, M * NYX)) || (rc = dev_alloc(&filt, M * NYX)) ||
       (rc = dev_alloc(&nsel, M)) || (rc = dev_alloc(&nout, M)) || (rc = dev_alloc(&flag, M * NYX)) ||
-      (rc = dev_alloc(&vst, (size_t)4 * M)) || (rc = dev_alloc(&vhist, (size_t)256 * M))) {
+      (rc = dev_alloc(&vst, (size_t)4 * M)) || (rc = dev_alloc(&vhist, (size_t)256 * M)) ||
+      (rc = dev_alloc(&all, NYX)) || (rc = dev_alloc(&cnt, (size_t)M * nch)) ||
+      (rc = dev_alloc(&list, M * NYX)) || (rc = dev_alloc(&count, M))) {
   } else {
     hipError_t e = hipMemcpy(mag, &MagUpdateMap[0][0][0][0], M * NYX * sizeof(float),
                              hipMemcpyHostToDevice);
@@ -2126,27 +2129,38 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
       hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((NYX + 255) / 256), M), dim3(256), 0, 0, NY,
                          NX, mag, filt);
       launch_vsc_select(0, (int)NYX, M, filt, nsel, flag, nout, vst, vhist);
+      e = hipMemset(all, 1, NYX);
+    }
+    if (e == hipSuccess) {
+      // the selected lines of every map in raster order, by the kernels that
+      // build the NHICD line lists of a reconstruction (mask: every line)
+      const dim3 gc((unsigned)nch, (unsigned)M);
+      hipLaunchKernelGGL(k_nhicd_chunk_count, gc, dim3(256), 0, 0, (int)NYX, nch, flag, all, cnt);
+      hipLaunchKernelGGL(k_nhicd_chunk_write, gc, dim3(256), 0, 0, (int)NYX, nch, flag, all, cnt,
+                         list, count);
       e = hipDeviceSynchronize();
     }
-    if (e == hipSuccess) e = hipMemcpy(hflag.data(), flag, M * NYX, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(hlist.data(), list, M * NYX * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hcount.data(), count, M * sizeof(int), hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = set_err(XT_E_HIP, "vsc seam: %s", hipGetErrorString(e));
   }
-  void *b[] = {mag, filt, nsel, nout, flag, vst, vhist};
+  void *b[] = {mag, filt, nsel, nout, flag, vst, vhist, all, cnt, list, count};
   for (void *p : b)
     if (p) (void)hipFree(p);
   if (rc) return rc;
+  for (int m = 0; m < M; m++)
+    if (hcount[m] < 0 || (size_t)hcount[m] > NYX)
+      return set_err(XT_E_HIP, "vsc seam: line count %d of map %d", hcount[m], m);
   for (int i = 0; i < O->N_time; i++)
     for (int bb = 0; bb < nb; bb++) {
-      const uint8_t *F = hflag.data() + (size_t)(i * nb + bb) * NYX;
-      int idx = 0;
-      for (int j = 0; j < NY; j++)
-        for (int k = 0; k < NX; k++)
-          if (F[(size_t)j * NX + k]) {
-            T->x_NHICD_select[i][bb][idx] = k;
-            T->y_NHICD_select[i][bb][idx] = j;
-            idx++;
-          }
-      T->NHICDSelectNum[i][bb] = idx;
+      const int m = i * nb + bb;
+      const int *L = hlist.data() + (size_t)m * NYX;
+      for (int idx = 0; idx < hcount[m]; idx++) {
+        T->x_NHICD_select[i][bb][idx] = L[idx] % NX;
+        T->y_NHICD_select[i][bb][idx] = L[idx] / NX;
+      }
+      T->NHICDSelectNum[i][bb] = hcount[m];
     }
   return XT_OK;
 }
