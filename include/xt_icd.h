@@ -264,6 +264,21 @@ int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, i
                     int32_t n_lines, const int32_t *x_list, const int32_t *y_list, int32_t Iter,
                     Real_t *MagUpdate, double *total_vox_mag, const uint8_t *Mask);
 
+/* The production form of the voxel-update pass on listed lines of a 3D slab
+ * (N_time = 1): the same reference functions as xt_update_lines, but as the
+ * ICD iterations run them -- the slices of the z-blocks of one parity (0: even
+ * blocks, 1: odd, F/XT_ICD_update.c:799-881) updated at once per line, with the
+ * error sinogram split by parity and those slices packed into half the lanes
+ * when the blocks are of equal size and even in number.  Lines in list order,
+ * one at a time.  Blocks of one parity share no sinogram entries and no prior
+ * term, so the result equals the reference's per-block loop over the same lines
+ * (each block of the parity in turn).  MagUpdate[N_y][N_x] receives the summed
+ * |change| of each line over the parity's blocks. */
+int xt_update_lines_parity(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                           int32_t n_lines, const int32_t *x_list, const int32_t *y_list,
+                           int32_t Iter, int32_t parity, Real_t *MagUpdate,
+                           double *total_vox_mag);
+
 /* edge_detect, F/XT_ImageProc.c:83-149: the edge-only update mask
  * Mask[N_time][N_y][N_x][N_z] (one contiguous block, as the reference's
  * multialloc) of O->Object: per frame and slice a 5x5 Gaussian blur, the 3x3

@@ -1285,6 +1285,11 @@ int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, con
         snprintf(name, sizeof name, "object_n%d_time_%d_iter_%d", T->node_rank, i, Iter);
         write_bin(dir, name, &(O->Object[i][1][0][0]), (size_t)O->N_z * O->N_y * O->N_x);
       }
+      {
+        char name[128];
+        snprintf(name, sizeof name, "proj_offset_n%d_iter_%d", T->node_rank, Iter);
+        write_bin(dir, name, &(S->ProjOffset[0][0]), (size_t)S->N_r * S->N_t);
+      }
       if (T->debug_file_ptr) fflush(T->debug_file_ptr);
     }
     if (flag != 0 && Iter > 1) break;

@@ -1,0 +1,106 @@
+"""Writes tests/golden/c1_cpu_icd.npz: the CPU ICD (the oracle's C restatement
+of the reference engine, float build, OpenMP) on the plain 3D workload
+(128x128x8, 256 views over [0,180), 192x8 bins, I0 = 2e4, 0.5% zingers, ring
+offsets; synth.C1) -- the object after 30 iterations and after --iters
+iterations (sampled at 32768 seeded voxel positions), the float build's MAP cost
+after every iteration, and the FP64 MAP cost (f64a build: the float system
+matrix in double arithmetic, e = y - Ax - d recomputed) of both objects with
+their ring offsets.  tests/test_gpu_configs.py runs the GPU engine on the same
+seeded scan and compares.
+
+    python tests/golden/make_c1_reference.py RUN_DIR [--iters 300]
+"""
+import argparse
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import conftest  # noqa: E402,F401  (registers the package)
+import py_oracle  # noqa: E402
+from timbir_python_amd import synth  # noqa: E402
+
+N_SAMPLE = 32768
+SEED = 20261203
+EARLY = 30
+
+ap = argparse.ArgumentParser()
+ap.add_argument("run_dir")
+ap.add_argument("--iters", type=int, default=300)
+a = ap.parse_args()
+cfg = synth.C1
+d = a.run_dir
+args = cfg.cli_args(iterations=a.iters)
+os.makedirs(d, exist_ok=True)
+final = os.path.join(d, f"object_n0_time_0_iter_{a.iters}.bin")
+if not os.path.exists(final):
+    if not os.path.exists(os.path.join(d, "weight_n0.bin")):
+        o64a = py_oracle.Oracle(True, float_amatrix=True)
+        synth.generate(cfg, d, o64a.projector(d, args))
+    o = py_oracle.Oracle(False)
+    n, arr = py_oracle.argv_array(args)
+    costs = np.full(a.iters + 1, np.nan)
+    it = C.c_int32()
+    t0 = time.time()
+    rc = o.lib.oracle_engine_run(d.encode(), n, arr, 1 | 4, costs.ctypes.data, C.byref(it))
+    assert rc == 0, rc
+    print("oracle ICD", it.value, "iterations in", time.time() - t0, "s", flush=True)
+    np.save(os.path.join(d, "costs_f32.npy"), costs)
+
+shape = (cfg.N_z, cfg.N_xy, cfg.N_xy)
+nvox = int(np.prod(shape))
+idx = np.sort(np.random.default_rng(SEED).choice(nvox, N_SAMPLE, replace=False)).astype(np.int32)
+ph = synth.phantom(cfg)[0]
+mean_att = float(ph[ph > 0].mean())
+
+
+def snap(k):
+    v = np.fromfile(os.path.join(d, f"object_n0_time_0_iter_{k}.bin"), np.float32)
+    off = np.fromfile(os.path.join(d, f"proj_offset_n0_iter_{k}.bin"), np.float32)
+    return v, off
+
+
+def map_cost64(vol, offset):
+    """FP64 MAP cost of (object, ring offsets) on this scan, f64a oracle."""
+    o64 = py_oracle.Oracle(True, float_amatrix=True)
+    d64 = d.rstrip("/") + "_f64"
+    os.makedirs(d64, exist_ok=True)
+    for f in ("projection_n0.bin", "weight_n0.bin"):
+        np.fromfile(os.path.join(d, f), np.float32).astype(np.float64).tofile(os.path.join(d64, f))
+    np.zeros(cfg.N_r * cfg.N_t).tofile(os.path.join(d64, "proj_offset_n0.bin"))
+    with open(os.path.join(d, "view_info.txt")) as src, \
+            open(os.path.join(d64, "view_info.txt"), "w") as dst:
+        dst.write(src.read())
+    S, O, T = o64.setup(d64, args)
+    try:
+        o64.read_sinogram(d64, S, T)
+        o64.set_object(O, vol.reshape((1,) + shape).astype(np.float64))
+        o64.offset_view(S)[:] = offset.reshape(cfg.N_r, cfg.N_t)
+        H = o64.detector_response(S, O, T)
+        E = np.zeros((S.N_p, S.N_r, S.N_t), np.float64)
+        et, ht = py_oracle.xt_abi.PtrTree(E), py_oracle.xt_abi.PtrTree(H)
+        o64.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+        return float(o64.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr))
+    finally:
+        o64.free(S, O, T)
+
+
+v30, d30 = snap(EARLY)
+vN, dN = snap(a.iters)
+c30, cN = map_cost64(v30, d30), map_cost64(vN, dN)
+# the oracle's own distance to its last iterate, as RMSE / mean attenuation
+own = np.array([np.sqrt(np.mean((snap(k)[0] - vN) ** 2)) / mean_att for k in range(1, a.iters)])
+costs = np.load(os.path.join(d, "costs_f32.npy"))
+np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "c1_cpu_icd.npz"),
+                    idx=idx, early_values=v30[idx], final_values=vN[idx],
+                    early_iterations=np.int32(EARLY), final_iterations=np.int32(a.iters),
+                    early_cost64=np.float64(c30), final_cost64=np.float64(cN),
+                    mean_att=np.float64(mean_att), costs_f32=costs, own_rmse_to_final=own,
+                    layout=np.array("[N_z][N_y][N_x] float32 sample, frame 0"))
+print(f"FP64 cost after {EARLY}: {c30!r}, after {a.iters}: {cN!r}; own RMSE to final at "
+      f"{EARLY}: {own[EARLY - 1]:.4%}, at {a.iters // 2}: {own[a.iters // 2 - 1]:.4%}", flush=True)

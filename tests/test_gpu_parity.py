@@ -26,6 +26,10 @@ pytestmark = pytest.mark.gpu
 # A columns of up to 8 and 14 bins, the line kernel's wide-column variant
 WIDE3 = dataclasses.replace(TINY, name="wide3", N_xy=16, mult_xy=3.0)
 WIDE6 = dataclasses.replace(TINY, name="wide6", N_xy=8, mult_xy=6.0)
+# the production slab's z shape (32 slices in 32 one-slice z-blocks: 32 z lanes,
+# 16 packed per parity) and the deepest slab a GPU takes (64 slices, 64 lanes)
+NZ32 = dataclasses.replace(TINY, name="nz32", N_xy=32, N_z=32, N_p=48, num_threads=32, seed=21)
+NZ64 = dataclasses.replace(TINY, name="nz64", N_xy=24, N_z=64, N_p=48, num_threads=64, seed=23)
 
 
 def _print_case(name, cfg):
@@ -99,7 +103,7 @@ def test_amatrix_columns_bitexact(cfg, scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
-@pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6], ids=lambda c: c.name)
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6, NZ32, NZ64], ids=lambda c: c.name)
 def test_forward_project(cfg, scan_root, oracle32, oracle64a):
     """Ax within 1e-5 relative of the float reference system matrix (FP64 sums).
     The reference's double build uses a different detector table (its float
@@ -221,15 +225,21 @@ def test_variance_update(scan_root, oracle64a):
 VSC96 = dataclasses.replace(TINY, name="vsc96", N_xy=96, N_p=48, seed=17)
 
 
-@pytest.mark.parametrize("cfg", [SMALL4D, VSC96], ids=lambda c: c.name)
-def test_vsc_select_bitexact(cfg, scan_root, oracle32):
+@pytest.mark.parametrize("cfg,zero_rows", [(SMALL4D, 3), (VSC96, 3), (VSC96, 90)],
+                         ids=["small4d", "vsc96", "vsc96-threshold-at-zero"])
+def test_vsc_select_bitexact(cfg, zero_rows, scan_root, oracle32):
+    """Line sets of the GPU selection against the oracle's (pinned to the
+    reference's compiled XT_NHICD.c in test_oracle_pins.py) and, where the
+    reference build is present, against XT_NHICD.c itself.  With 90 of 96 rows
+    zero the map's top tenth reaches into the zero-valued lines, so the
+    threshold falls on a tie at zero."""
     _print_case("vsc_select", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     try:
         nb = T.num_z_blocks
         rng = np.random.default_rng(7)
         mag = rng.exponential(1e-4, (O.N_time, nb, O.N_y, O.N_x)).astype(np.float32)
-        mag[..., :3, :] = 0  # ties at zero
+        mag[..., :zero_rows, :] = 0  # ties at zero
         usn = np.full((O.N_time, nb), O.N_y * O.N_x, np.int32)
         got = xt_hip.vsc_select(eng, mag, usn)
         usn_o = np.ctypeslib.as_array((C.c_int32 * (O.N_time * nb)).from_address(
@@ -242,20 +252,34 @@ def test_vsc_select_bitexact(cfg, scan_root, oracle32):
         num = py_oracle.deref_data(T.NHICDSelectNum, 1, C.c_int32, O.N_time * nb).reshape(O.N_time, nb)
         xs = py_oracle.deref_data(T.x_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
         ys = py_oracle.deref_data(T.y_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
-        for i in range(O.N_time):
-            for b in range(nb):
-                k = num[i, b]
-                gy, gx = got[i][b]
-                assert len(gy) == k
-                assert np.array_equal(gy, ys[i, b, :k]) and np.array_equal(gx, xs[i, b, :k])
+        sets = [(num.copy(), xs.copy(), ys.copy())]
+        if py_oracle.RefLib.available():
+            ref = py_oracle.RefLib(False)
+            ref.lib.srandom2(761521)
+            mt_ref = xt_abi.PtrTree(mag.copy())  # kept alive across the call
+            ref.lib.VSC_based_Voxel_Line_Select(C.byref(O), C.byref(T), mt_ref.ptr)
+            sets.append((num.copy(), xs.copy(), ys.copy()))
+        for n_o, x_o, y_o in sets:
+            for i in range(O.N_time):
+                for b in range(nb):
+                    k = n_o[i, b]
+                    gy, gx = got[i][b]
+                    assert len(gy) == k
+                    assert np.array_equal(gy, y_o[i, b, :k]) and np.array_equal(gx, x_o[i, b, :k])
+        if zero_rows > O.N_y // 2:
+            # the threshold is 0 (a tie): only lines strictly above it are taken
+            # (F/XT_NHICD.c:189), fewer than the requested share
+            assert 0 < num.max() < (O.N_y * O.N_x) // O.NHICD_Iterations
     finally:
         eng.close()
         oracle32.free(S, O, T)
 
 
 @pytest.mark.parametrize("cfg,atable,nt", [(TINY, 1, 0), (TINY, 0, 0), (WIDE3, 0, 0), (WIDE6, 0, 0),
-                                            (TINY, 1, 64), (WIDE3, 0, 64)],
-                         ids=["atable", "onthefly", "wide3", "wide6", "atable-wave", "wide3-wave"])
+                                            (TINY, 1, 64), (WIDE3, 0, 64), (NZ32, 1, 0),
+                                            (NZ64, 1, 0)],
+                         ids=["atable", "onthefly", "wide3", "wide6", "atable-wave", "wide3-wave",
+                              "nz32", "nz64"])
 @pytest.mark.parametrize("Iter", [1, 2])
 def test_update_lines_per_voxel(Iter, cfg, atable, nt, scan_root, oracle32):
     _print_case(f"update_lines Iter={Iter} atable={atable} threads={nt or 256}", cfg)
@@ -288,6 +312,73 @@ def test_update_lines_per_voxel(Iter, cfg, atable, nt, scan_root, oracle32):
         xg = eng.volume()
         scale = np.abs(xo).max()
         print(f"  max |dx| {np.abs(xg - xo).max() / scale:.3e}  e {np.abs(Eg - E).max():.3e}")
+        assert np.abs(xg - xo).max() <= 1e-4 * scale
+        assert np.abs(Eg - E).max() <= 1e-4 * np.abs(E).max()
+        assert np.abs(mag_g - mag_o).max() <= 1e-4 * max(1e-12, np.abs(mag_o).max())
+        assert abs(tot_g - tot_o) <= 1e-4 * max(1e-12, abs(tot_o))
+    finally:
+        xt_hip.set_option(xt_hip.OPT_ATABLE, 1)
+        xt_hip.set_option(xt_hip.OPT_LINE_THREADS, 0)
+        eng.close()
+        oracle32.free(S, O, T)
+
+
+@pytest.mark.parametrize("cfg,atable,nt", [(NZ32, 1, 256), (NZ32, 0, 256), (NZ32, 1, 64),
+                                            (NZ64, 1, 256), (SMALL, 1, 256)],
+                         ids=["nz32-atable", "nz32-onthefly", "nz32-wave", "nz64-atable",
+                              "small-atable"])
+@pytest.mark.parametrize("Iter", [1, 2])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_update_lines_parity_per_voxel(parity, Iter, cfg, atable, nt, scan_root, oracle32):
+    """The pass the ICD iterations run -- one z-block parity, the sinogram split by
+    parity, the parity's slices packed into half the lanes (k_icd_lines<32, TAB,
+    PACK=true, 6, 256> for the production slab's 32 one-slice blocks) -- against
+    the oracle's sequential reference loop over the same lines, block by block of
+    that parity (F/XT_ICD_update.c:799-881, F/XT_VoxUpdate.c:85-254)."""
+    _print_case(f"update_lines_parity parity={parity} Iter={Iter} atable={atable} threads={nt}",
+                cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    xt_hip.set_option(xt_hip.OPT_ATABLE, atable)
+    xt_hip.set_option(xt_hip.OPT_LINE_THREADS, nt)
+    try:
+        nb = T.num_z_blocks
+        bs = O.N_z // nb
+        assert nb % 2 == 0 and O.N_z % nb == 0
+        vol = (0.7 * info["phantom"]).astype(np.float32)
+        vol[:, :, :4, :] = 0  # zero rows: zero-skipping paths
+        oracle32.set_object(O, vol)
+        eng.set_volume(vol)
+        H = oracle32.detector_response(S, O, T)
+        E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
+        et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+        oracle32.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+        Eg = E.copy()
+        rng = np.random.default_rng(Iter + 10 * parity)
+        n = 40
+        ys = rng.integers(0, O.N_y, n).astype(np.int32)
+        xs = rng.integers(0, O.N_x, n).astype(np.int32)
+        ys[:4] = 1  # lines in the zero rows
+        mag_o = np.zeros((O.N_y, O.N_x), np.float32)
+        tot_o = 0.0
+        for b in range(parity, nb, 2):
+            mb = np.zeros((O.N_y, O.N_x), np.float32)
+            mt = xt_abi.PtrTree(mb)
+            zc = C.c_int64(0)
+            tot_o += oracle32.lib.oracle_update_lines(
+                C.byref(S), C.byref(O), C.byref(T), et.ptr, ht.ptr, 0, b * bs, (b + 1) * bs - 1,
+                n, xs.ctypes.data, ys.ctypes.data, Iter, C.byref(zc), mt.ptr, 0)
+            mag_o += mb
+        mag_g, tot_g = xt_hip.update_lines_parity(eng, Eg, xs, ys, Iter, parity)
+        xo = oracle32.get_object(O)
+        xg = eng.volume()
+        scale = np.abs(xo).max()
+        moved = np.abs(xo - vol).max() / scale
+        print(f"  max |dx| {np.abs(xg - xo).max() / scale:.3e} (moved {moved:.3e})  "
+              f"e {np.abs(Eg - E).max():.3e}")
+        assert moved > 1e-3
+        # slices of the other parity are not touched
+        other = np.array([((z // bs) & 1) != parity for z in range(O.N_z)])
+        assert np.array_equal(xg[:, other], vol[:, other])
         assert np.abs(xg - xo).max() <= 1e-4 * scale
         assert np.abs(Eg - E).max() <= 1e-4 * np.abs(E).max()
         assert np.abs(mag_g - mag_o).max() <= 1e-4 * max(1e-12, np.abs(mag_o).max())

@@ -164,3 +164,52 @@ def test_output_writers(tmp_path):
     assert len(names) == 6
     img = np.asarray(Image.open(tmp_path / "tif" / "object_time_z_1_2.tif"))
     assert img.dtype == np.uint8 and img[1, 1] == 255 and img[0, 0] == 127
+
+
+REF_F = "/root/reference/Source_Code_4D_Fast"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_F), reason="reference sources absent")
+def test_dropin_links_with_reference_host_objects():
+    """The reference host sources that stay on the engine's link line when the
+    library replaces XT_ICD_update.c, XT_VoxUpdate.c, XT_NHICD.c and
+    XT_ForwardProject.c (XT_AMatrix.c and XT_Profile.c stay: XT_genSinogram.c:135,
+    144,168 call them) link against libxt_hip.so with --no-undefined, and
+    ICD_BackProject resolves from the library."""
+    r = subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "linkcheck",
+                        f"REF={REF_F}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    so = os.path.join(ROOT, "oracle", "_ref", "libxt_host_link.so")
+    dyn = subprocess.run(["readelf", "-d", so], capture_output=True, text=True).stdout
+    assert "[libxt_hip.so]" in dyn
+    und = subprocess.run(["nm", "-D", "-u", so], capture_output=True, text=True).stdout.split()
+    assert "ICD_BackProject" in und
+    # without XT_AMatrix.c and XT_Profile.c the link must fail on exactly their symbols
+    r = subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "linkcheck",
+                        f"REF={REF_F}", "HOST_SRCS=XT_genSinogram.c XT_IOMisc.c XT_ImageProc.c "
+                        "XT_Filter.c XT_HDFIO.c allocate.c randlib.c tiff.c"],
+                       capture_output=True, text=True)
+    assert r.returncode != 0
+    missing = set(re.findall(r"undefined reference to `(\w+)'", r.stderr))
+    assert missing == {"DetectorResponseProfile", "storeVoxelLineResponse",
+                       "calcAMatrixColumnforAngle"}
+    subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "linkcheck",
+                    f"REF={REF_F}"], capture_output=True, text=True, check=True)
+
+
+@pytest.mark.parametrize("n_p,n_r,n_t", [(16384, 2048, 16), (8192, 2048, 32), (4096, 8192, 16)])
+def test_oversize_sinogram_refused(n_p, n_r, n_t):
+    """The voxel-update kernels address the sinogram through 32-bit byte offsets
+    with 2^31 as the absent-bin value, so a sinogram of 2^29 entries or more is an
+    argument error, raised before any device call (here: with no GPU)."""
+    S, O, T = xt_hip.Sinogram(), xt_hip.ScannedObject(), xt_hip.TomoInputs()
+    S.N_p, S.N_r, S.N_t, S.z_overlap_num, S.delta_r = n_p, n_r, n_t, 1, 1.0
+    O.N_x = O.N_y = n_r * 2 // 3
+    O.N_z, O.N_time, O.delta_xy = n_t, 1, 1.5
+    assert n_p * n_r * n_t >= 1 << 29
+    lib = xt_hip.lib()
+    state = C.c_void_p()
+    rc = lib.xt_icd_create(C.byref(S), C.byref(O), C.byref(T), C.byref(state))
+    assert rc == 1  # XT_E_ARG
+    assert b"at most 2^29 - 1 entries" in lib.xt_last_error()
+    assert not state.value

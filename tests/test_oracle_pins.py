@@ -64,6 +64,25 @@ def test_interlaced_views_golden_log(oracle32):
     assert dv[0] == 3080 * 180.0 / Nt and dt[0] == 3080 / 32
 
 
+def test_synth_launcher_views_golden_log():
+    """The product generator's launcher view list (interlaced order, then
+    clip_list_of_views) reproduces the 399 deleted views and times of info.txt:1-175
+    (K=32, N_theta=1536, rotation 100 deg/s, 0.03 s between views: 3 degrees), and
+    keeps everything else in order."""
+    gold = json.load(open(os.path.join(GOLD, "interlaced_clip.json")))
+    K, Nt = gold["K"], gold["N_theta"]
+    kv, kt, dv, dt = synth.launcher_views(K, Nt, 25, 0.03, 100.0)
+    assert len(dv) == 399 and len(kv) == Nt * 25 - 399
+    assert np.array_equal(dv, np.array(gold["deleted_views"]))
+    assert np.array_equal(dt, np.array(gold["deleted_times"]))
+    ang, t = synth.interlaced_views(K, Nt, Nt * 25)
+    assert np.array_equal(np.sort(np.concatenate([kv, dv])), np.sort(ang))
+    assert kt[0] == 0.0 and np.array_equal(kt[1:], t[np.isin(ang, kv)][1:])
+    # no clipping below the smallest step: every view kept
+    kv0, _, dv0, _ = synth.launcher_views(K, Nt, 2, 0.0, 100.0)
+    assert len(dv0) == 0 and len(kv0) == 2 * Nt
+
+
 def test_synth_interlaced_views_match_oracle(oracle32):
     cfg = dataclasses.replace(synth.C3, N_p=1000)
     ang, t = cfg.views()
@@ -246,18 +265,26 @@ def test_forward_project_vs_reference_scatter(oracle32, scan_root):
 
 
 # -- NHICD selection ---------------------------------------------------------
+VSC96 = dataclasses.replace(TINY, name="vsc96", N_xy=96, N_p=48, seed=17)
+
+
 @needs_ref
-def test_vsc_select_bitexact_vs_reference(oracle32, scan_root):
-    """Same line sets and the same random numbers consumed (F/XT_NHICD.c:164-216)."""
+@pytest.mark.parametrize("cfg,zero_rows,full", [(SMALL4D, 2, False), (VSC96, 3, True),
+                                                (VSC96, 90, True)],
+                         ids=["small4d", "vsc96", "vsc96-threshold-at-zero"])
+def test_vsc_select_bitexact_vs_reference(cfg, zero_rows, full, oracle32, scan_root):
+    """Same line sets and the same random numbers consumed (F/XT_NHICD.c:164-216),
+    on 32x32 maps of four frames and on 96x96 maps, one with 90 of 96 rows zero so
+    that the order statistic is a tie at zero."""
     ref = py_oracle.RefLib(False)
-    path, info, (S, O, T) = _setup(oracle32, SMALL4D, scan_root)
+    path, info, (S, O, T) = _setup(oracle32, cfg, scan_root)
     try:
         nb, nxy = T.num_z_blocks, O.N_y * O.N_x
         rng = np.random.default_rng(3)
         mag = rng.exponential(1.0, (O.N_time, nb, O.N_y, O.N_x)).astype(np.float32)
-        mag[:, :, :2] = 0
+        mag[:, :, :zero_rows] = 0
         usn = py_oracle.deref_data(T.UpdateSelectNum, 1, C.c_int32, O.N_time * nb)
-        usn[:] = rng.integers(nxy // 2, nxy + 1, usn.size)
+        usn[:] = nxy if full else rng.integers(nxy // 2, nxy + 1, usn.size)
         results = []
         for fn, seed in ((oracle32.lib.oracle_vsc_select, oracle32.lib.oracle_srandom2),
                          (ref.lib.VSC_based_Voxel_Line_Select, ref.lib.srandom2)):

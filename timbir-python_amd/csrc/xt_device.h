@@ -442,7 +442,9 @@ __device__ __forceinline__ float xt_qggmrf_ratio(float delta, float p, float c, 
 // Per-launch work description of the voxel-update kernel.
 struct XtIcdLaunch {
   int mode;        // 0 homogeneous share, 1 NHICD lines, 2 the given list in order (one
-                   // workgroup; per-voxel checks)
+                   // workgroup, every z-block; per-voxel checks), 3 the given list in
+                   // order, the z-blocks of parity half only (per-voxel checks of the
+                   // production pass)
   int half;        // 0/1: the reference's even and odd z-block passes: frame f updates
                    // the slices of z-blocks with parity half ^ (f & 1)
                    // (F/XT_ICD_update.c:799-881); each is followed by a ring-offset update

@@ -63,6 +63,9 @@ int xt_report_error(int code, const char *fmt, ...) {
   } while (0)
 
 static double g_opt[18] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0};  // XT_OPT_*
+// bumped whenever an option changes value: a captured iteration graph (which
+// froze the options it was recorded under) is recaptured when it differs
+static uint64_t g_opt_gen = 1;
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
@@ -72,6 +75,7 @@ extern "C" int xt_set_option(int option, double value) {
   if (option < 1 || option > 17) return set_err(XT_E_ARG, "unknown option %d", option);
   if (option == XT_OPT_LINE_THREADS && value != 0 && value != 64 && value != 256)
     return set_err(XT_E_ARG, "XT_OPT_LINE_THREADS must be 0, 64 or 256");
+  if (g_opt[option] != value) g_opt_gen++;
   g_opt[option] = value;
   return XT_OK;
 }
@@ -137,6 +141,12 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
       O->N_time <= 0)
     return set_err(XT_E_ARG, "empty problem (N_p=%d N_r=%d N_t=%d N_x=%d N_y=%d N_z=%d N_time=%d)",
                    S->N_p, S->N_r, S->N_t, O->N_x, O->N_y, O->N_z, O->N_time);
+  // the voxel-update kernels address e and w through 32-bit buffer offsets, and an
+  // offset of 2^31 is their "absent bin" value (reads 0, drops the atomic): every
+  // byte offset into the sinogram must stay below it
+  if ((size_t)S->N_p * (size_t)S->N_r * (size_t)S->N_t * sizeof(float) >= ((size_t)1 << 31))
+    return set_err(XT_E_ARG, "sinogram of %d x %d x %d entries: at most 2^29 - 1 entries per GPU",
+                   S->N_p, S->N_r, S->N_t);
   if (S->z_overlap_num != 1 || S->N_t != O->N_z)
     return set_err(XT_E_ARG, "z_overlap_num=%d (N_t=%d, N_z=%d): only delta_z = 1 is supported",
                    S->z_overlap_num, S->N_t, O->N_z);
@@ -242,7 +252,8 @@ struct xt_icd_state {
   uint8_t *line_mask = nullptr;
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
   float *mag_blk = nullptr, *magsum = nullptr, *filt = nullptr, *magpar = nullptr;
-  int *nsel = nullptr, *nsel_out = nullptr;
+  int *nsel = nullptr, *nsel_out = nullptr;  // NHICD lines per (frame, z-block) map
+  int *nsel_par = nullptr;  // NHICD lines per (frame, z-block parity) map
   unsigned *vsc_st = nullptr, *vsc_hist = nullptr;  // NHICD radix selection scratch
   double *stats = nullptr, *red_wg = nullptr, *scal = nullptr;
   double *off_part = nullptr, *off_numden = nullptr, *off_wg = nullptr;
@@ -255,6 +266,8 @@ struct xt_icd_state {
   uint8_t *vmask = nullptr;     // edge-only: [N_time][N_y][N_x][N_z] voxels to update
   hipGraph_t graph = nullptr;   // one captured iteration (iterations >= 2)
   hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_opt_gen = 0;   // g_opt_gen when the graph was captured
+  uint64_t lines_gen = 1;       // bumped when the line list or its NHICD shares change
   int64_t graph_launches = 0;   // voxel-update launches in the captured iteration
   int off_nch = 1, off_chunk = 1, off_nblk = 1;
   // statistics
@@ -274,6 +287,7 @@ static void free_state(xt_icd_state *s) {
                   s->lines,    s->nhicd_list, s->nhicd_count, s->nhicd_chunk, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
+                  s->nsel_par,
                   s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
                   s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp, s->vmask, s->vsc_st,
@@ -339,7 +353,7 @@ static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *
 // equal size and an even number of them.
 static bool icd_packable(const XtGeom &g, int ZL, const XtIcdLaunch &L) {
   (void)ZL;
-  return g.pm != 0 && L.mode != 2;
+  return g.pm != 0 && L.mode != 2;  // mode 2 updates every block's slices
 }
 
 template <int ZL, int NT>
@@ -509,7 +523,7 @@ static int vsc_select(xt_icd_state *s) {
     dim3 gf((unsigned)((s->NYX + 255) / 256), M);
     hipLaunchKernelGGL(k_vsc_filter, gf, dim3(256), 0, s->st, s->g.N_y, s->g.N_x, s->magpar,
                        s->filt);
-    launch_vsc_select(s->st, (int)s->NYX, M, s->filt, s->nsel, s->nhicd_flag, s->nsel_out,
+    launch_vsc_select(s->st, (int)s->NYX, M, s->filt, s->nsel_par, s->nhicd_flag, s->nsel_out,
                       s->vsc_st, s->vsc_hist);
     hipLaunchKernelGGL(k_flag_blocks, dim3(nl), dim3(256), 0, s->st, s->g.nb, s->NYX,
                        s->g.N_time, s->nhicd_flag, s->nhicd_blk);
@@ -904,27 +918,44 @@ static int select_edges(xt_icd_state *s) {
   XT_TRY(launch_edge_mask(s, s->x, s->vmask));
   std::vector<uint8_t> m(s->nobj);
   XT_HIP(hipMemcpy(m.data(), s->vmask, s->nobj, hipMemcpyDeviceToHost));
-  const int nb = g.nb, bs = g.N_z / g.nb;
-  std::vector<int> lines, ns((size_t)g.N_time * (nb > 2 ? nb : 2), 0);
+  // per (frame, z-block) map: that block's lines with a masked voxel; per (frame,
+  // parity) map (XT_OPT_NHICD_LINES = 1, where a selected line is updated in every
+  // block of the parity): the lines with a masked voxel in some block of the parity
+  const int nb = g.nb;
+  auto zblock = [&](int z) {  // the line kernel's block of slice z
+    const int bs = g.N_z / g.nb;
+    const int b = bs > 0 ? z / bs : g.nb - 1;
+    return b >= g.nb ? g.nb - 1 : b;
+  };
+  std::vector<int> lines, ns((size_t)g.N_time * (nb > 2 ? nb : 2), 0), np((size_t)g.N_time * 2, 0);
+  std::vector<uint8_t> has(nb);
   for (size_t c = 0; c < s->NYX; c++) {
     bool any = false;
     for (int f = 0; f < g.N_time; f++) {
       const uint8_t *v = &m[((size_t)f * s->NYX + c) * g.N_z];
-      for (int z = 0; z < g.N_z; z++) any |= v[z] != 0;
+      std::fill(has.begin(), has.end(), 0);
+      for (int z = 0; z < g.N_z; z++)
+        if (v[z]) has[zblock(z)] = 1;
+      bool par_has[2] = {false, false};
       for (int b = 0; b < nb; b++) {
-        bool has = false;
-        for (int z = b * bs; z < (b + 1) * bs; z++) has |= v[z] != 0;
-        ns[(size_t)f * nb + b] += has;
+        ns[(size_t)f * nb + b] += has[b];
+        par_has[b & 1] |= has[b] != 0;
+        any |= has[b] != 0;
       }
+      np[(size_t)f * 2] += par_has[0];
+      np[(size_t)f * 2 + 1] += par_has[1];
     }
     if (any) lines.push_back((int)c);
   }
   for (int &n : ns) n /= s->K;
+  for (int &n : np) n /= s->K;
   s->n_masked = (int)lines.size();
   if (!lines.empty())
     XT_HIP(hipMemcpy(s->lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_HIP(hipMemcpy(s->nsel_par, np.data(), np.size() * sizeof(int), hipMemcpyHostToDevice));
   drop_graph(s);
+  s->lines_gen++;
   return XT_OK;
 }
 
@@ -935,9 +966,11 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     free_state(s);
     return rc;
   };
-  int rc = init_device(&s->dev);
+  // the geometry is checked before any device call, so a bad one is refused the
+  // same way with or without a GPU
+  int rc = make_geom(S, O, T, &s->g);
   if (rc) return fail(rc);
-  rc = make_geom(S, O, T, &s->g);
+  rc = init_device(&s->dev);
   if (rc) return fail(rc);
   if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_err(XT_E_HIP, "hipStreamCreate"));
   XtGeom &g = s->g;
@@ -958,9 +991,6 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
                         g.max_views));
   s->nsino = (size_t)g.N_p * g.N_r * g.N_t;
   s->NYX = (size_t)g.N_y * g.N_x;
-  // the voxel update addresses e and w through 32-bit buffer offsets below 2^31
-  if (s->nsino * sizeof(float) >= ((size_t)1 << 31))
-    return fail(set_err(XT_E_ARG, "sinogram of %zu entries: at most 2^29 - 1", s->nsino));
   s->nobj = (size_t)g.N_time * s->NYX * g.N_z;
   // masked lines, F/XT_ICD_update.c:958-974
   std::vector<int> masked;
@@ -990,6 +1020,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
       (rc = dev_alloc(&s->filt, (size_t)g.N_time * (g.nb > 2 ? g.nb : 2) * s->NYX)) ||
       (rc = dev_alloc(&s->magpar, (size_t)g.N_time * 2 * s->NYX)) ||
       (rc = dev_alloc(&s->nsel, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
+      (rc = dev_alloc(&s->nsel_par, g.N_time * 2)) ||
       (rc = dev_alloc(&s->nsel_out, g.N_time * (g.nb > 2 ? g.nb : 2))) ||
       (rc = dev_alloc(&s->vsc_st, (size_t)4 * g.N_time * (g.nb > 2 ? g.nb : 2))) ||
       (rc = dev_alloc(&s->vsc_hist, (size_t)256 * g.N_time * (g.nb > 2 ? g.nb : 2))) ||
@@ -1028,7 +1059,9 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     // NHICD lines per (frame, z-block): floor(UpdateSelectNum / NHICD_Iterations),
     // F/XT_NHICD.c:182 (every block holds all masked lines)
     std::vector<int> ns((size_t)g.N_time * (g.nb > 2 ? g.nb : 2), s->n_masked / s->K);
-    if (hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->nsel_par, ns.data(), (size_t)g.N_time * 2 * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(XT_E_HIP, "nsel upload"));
   }
   if ((rc = upload_object(s, O, s->x))) return fail(rc);
@@ -1173,7 +1206,9 @@ static int iterate_one(xt_icd_state *s) {
   // once and replay it (no memset nodes; the iteration number lives on the device)
   const bool use_graph = g_opt[XT_OPT_GRAPH] != 0 && Iter > 1 && !s->timing;
   if (use_graph) {
+    if (s->graph_exec && s->graph_opt_gen != g_opt_gen) drop_graph(s);
     if (!s->graph_exec) {
+      s->graph_opt_gen = g_opt_gen;
       XT_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
       const int64_t l0 = s->launches;
       int rc = enqueue_iteration(s, nhicd);
@@ -1345,6 +1380,9 @@ struct xt_icd_batch {
   };
   std::vector<Pass> passes;  // [pass][group], pass = sub_iter * 4 + (nhicd ? 2 : 0) + half
   XtIcdSlab *tab = nullptr;
+  // g_opt_gen and the slabs' lines_gen the plan and the graph were made for
+  uint64_t opt_gen = 0;
+  std::vector<uint64_t> lines_gen;
 };
 
 // Groups of slabs that share a kernel variant, and the per-pass tables of the
@@ -1512,6 +1550,8 @@ extern "C" void xt_icd_batch_destroy(xt_icd_batch *b) {
   delete b;
 }
 
+static int batch_refresh(xt_icd_batch *b);
+
 extern "C" int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batch **batch) {
   if (!batch || n <= 0 || !states) return set_err(XT_E_ARG, "bad batch arguments");
   *batch = nullptr;
@@ -1535,7 +1575,7 @@ extern "C" int xt_icd_batch_create(int32_t n, xt_icd_state **states, xt_icd_batc
     xt_icd_batch_destroy(b);
     return set_err(XT_E_HIP, "batch streams/events");
   }
-  int rc = batch_plan(b);
+  int rc = batch_refresh(b);
   if (rc != XT_OK) {
     xt_icd_batch_destroy(b);
     return rc;
@@ -1587,9 +1627,31 @@ static int batch_capture(xt_icd_batch *b, const std::vector<int> &iters) {
   return XT_OK;
 }
 
+// Drop the batch's graph and fused tables when an option or a slab's line list
+// changed since they were made, and plan again.
+static int batch_refresh(xt_icd_batch *b) {
+  bool stale = b->opt_gen != g_opt_gen || b->lines_gen.size() != b->s.size();
+  for (size_t i = 0; !stale && i < b->s.size(); i++) stale = b->lines_gen[i] != b->s[i]->lines_gen;
+  if (!stale) return XT_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (b->tab) (void)hipFree(b->tab);
+  b->tab = nullptr;
+  b->groups.clear();
+  b->passes.clear();
+  XT_TRY(batch_plan(b));
+  b->opt_gen = g_opt_gen;
+  b->lines_gen.resize(b->s.size());
+  for (size_t i = 0; i < b->s.size(); i++) b->lines_gen[i] = b->s[i]->lines_gen;
+  return XT_OK;
+}
+
 extern "C" int xt_icd_batch_iterate(xt_icd_batch *b, int32_t n_iter) {
   if (!b) return set_err(XT_E_ARG, "null batch");
   for (int it = 0; it < n_iter; it++) {
+    XT_TRY(batch_refresh(b));
     std::vector<int> iters(b->s.size());
     bool graphable = g_opt[XT_OPT_GRAPH] != 0;
     for (size_t i = 0; i < b->s.size(); i++) {
@@ -1642,6 +1704,16 @@ extern "C" int xt_icd_batch_iterate(xt_icd_batch *b, int32_t n_iter) {
 static int write_run_bin(const char *name, const void *data, size_t count) {
   std::string p = run_path(name);
   FILE *fp = fopen(p.c_str(), "wb");
+  if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
+  size_t w = fwrite(data, sizeof(float), count, fp);
+  fclose(fp);
+  if (w != count) return set_err(XT_E_IO, "short write to %s", p.c_str());
+  return XT_OK;
+}
+
+static int append_run_bin(const char *name, const void *data, size_t count) {
+  std::string p = run_path(name);
+  FILE *fp = fopen(p.c_str(), "ab");
   if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
   size_t w = fwrite(data, sizeof(float), count, fp);
   fclose(fp);
@@ -1743,6 +1815,19 @@ extern "C" int xt_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, in
   return XT_OK;
 }
 
+// object_n<rank>_time_<i>.bin and proj_offset_n<rank>.bin of the current
+// structs, the binary part of write_ObjectProjOff2TiffBinPerIter
+// (F/XT_IOMisc.c:224-254)
+static int write_object_offset(Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  char name[128];
+  for (int i = 0; i < O->N_time; i++) {
+    snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
+    XT_TRY(write_run_bin(name, &O->Object[i][1][0][0], (size_t)O->N_z * O->N_y * O->N_x));
+  }
+  snprintf(name, sizeof name, "proj_offset_n%d.bin", T->node_rank);
+  return write_run_bin(name, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
+}
+
 extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   if (!S || !O || !T) return set_err(XT_E_ARG, "null argument");
   char name[256];
@@ -1791,12 +1876,29 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   xt_multifree(S->Projection, 3);
   S->Projection = nullptr;
   int rc = XT_OK;
+  // the MAP cost before ICD and after every iteration into cost_n<rank>.bin, one
+  // Real_t each (F/XT_ICD_update.c:1001-1007,1024-1027; the reference compiles this
+  // in without NO_COST_CALCULATE)
+  char costfile[64];
+  snprintf(costfile, sizeof costfile, "cost_n%d.bin", T->node_rank);
   if (g_opt[XT_OPT_COST] != 0) {
     rc = xt_icd_cost(s, &s->last_cost);
     dbg(T, "ICD_BackProject: cost before ICD %.9g\n", s->last_cost);
+    const float c0 = (float)s->last_cost;
+    if (rc == XT_OK) rc = write_run_bin(costfile, &c0, 1);
   }
   for (int Iter = 1; rc == XT_OK && Iter <= T->NumIter; Iter++) {
     rc = iterate_one(s);
+    // WritePerIter: the object and the ring offsets after every iteration
+    // (write_ObjectProjOff2TiffBinPerIter, F/XT_ICD_update.c:1019-1020)
+    if (rc == XT_OK && T->WritePerIter == 1) {
+      rc = xt_icd_download(s, S, O, T);
+      if (rc == XT_OK) rc = write_object_offset(S, O, T);
+    }
+    if (rc == XT_OK && g_opt[XT_OPT_COST] != 0) {
+      const float c = (float)s->last_cost;
+      rc = append_run_bin(costfile, &c, 1);
+    }
     if (rc == XT_OK && s->stopped && Iter > 1) {
       dbg(T, "ICD_BackProject: Convergence criteria is met\n");
       break;
@@ -1829,8 +1931,8 @@ struct SeamCtx {
 
 static int seam_state(SeamCtx &c, Sinogram *S, ScannedObject *O, TomoInputs *T) {
   c.s = new xt_icd_state();
-  XT_TRY(init_device(&c.s->dev));
   XT_TRY(make_geom(S, O, T, &c.s->g));
+  XT_TRY(init_device(&c.s->dev));
   XT_HIP(hipStreamCreate(&c.s->st));
   c.s->ZL = zl_for(c.s->g.N_z);
   c.s->nt = ((int)g_opt[XT_OPT_LINE_THREADS] == 64 && !c.s->g.pc) ? 64 : 256;
@@ -2165,10 +2267,15 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
   return XT_OK;
 }
 
-extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
-                               int32_t frame, int32_t n_lines, const int32_t *x_list,
-                               const int32_t *y_list, int32_t Iter, Real_t *MagUpdate,
-                               double *total_vox_mag, const uint8_t *Mask) {
+// The listed lines in list order by one workgroup: every slice, ascending
+// (parity < 0, launch mode 2), or the slices of one z-block parity at once with
+// the production form of the pass (parity 0/1, mode 3: the sinogram split by
+// parity and the parity's slices packed into half the lanes when the blocks
+// allow it, as xt_icd_create sets up).
+static int update_lines_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                             int32_t frame, int32_t n_lines, const int32_t *x_list,
+                             const int32_t *y_list, int32_t Iter, Real_t *MagUpdate,
+                             double *total_vox_mag, const uint8_t *Mask, int parity) {
   if (!S || !O || !T || !E || n_lines < 0 || (n_lines && (!x_list || !y_list)))
     return set_err(XT_E_ARG, "bad argument");
   if (frame < 0 || frame >= O->N_time) return set_err(XT_E_ARG, "frame %d", frame);
@@ -2176,7 +2283,13 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   SeamCtx c;
   XT_TRY(seam_state(c, S, O, T));
   xt_icd_state *s = c.s;
-  const XtGeom &g = s->g;
+  XtGeom &g = s->g;
+  if (parity >= 0 && g_opt[XT_OPT_PACK] != 0 && !g.pc && g.nb % 2 == 0 && g.N_z % g.nb == 0 &&
+      g.N_t % 2 == 0) {
+    g.pm = 1;
+    g.bs = g.N_z / g.nb;
+    g.H = g.N_t / 2;
+  }
   if (icd_lds_bytes(g, s->ZL, false) > 64 * 1024) return set_err(XT_E_ARG, "too many views");
   std::vector<int> cols(n_lines);
   for (int i = 0; i < n_lines; i++) {
@@ -2198,6 +2311,19 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(upload_object(s, O, s->x));
   XT_TRY(seam_upload_sino(s, &s->e, &E[0][0][0]));
   XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
+  if (g.pm) {
+    // the split layout of w and e: upload_sino stages each host array through e
+    float *w_split = nullptr, *e_split = nullptr;
+    XT_TRY(dev_alloc(&w_split, s->nsino));
+    int rc = upload_sino(s, w_split, &T->Weight[0][0][0]);
+    if (rc == XT_OK) rc = dev_alloc(&e_split, s->nsino);
+    if (rc == XT_OK) rc = upload_sino(s, e_split, &E[0][0][0]);
+    (void)hipFree(s->w);
+    (void)hipFree(s->e);
+    s->w = w_split;
+    s->e = e_split;
+    XT_TRY(rc);
+  }
   if (Mask) {
     XT_TRY(dev_alloc(&s->vmask, s->nobj));
     XT_HIP(hipMemcpy(s->vmask, Mask, s->nobj, hipMemcpyHostToDevice));
@@ -2207,6 +2333,7 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(dev_alloc(&s->nhicd_blk, (size_t)g.N_time * g.nb * s->NYX));
   XT_HIP(hipMemset(s->nhicd_blk, 1, (size_t)g.N_time * g.nb * s->NYX));
   XT_TRY(dev_alloc(&s->mag_blk, (size_t)g.N_time * g.nb * s->NYX));
+  XT_HIP(hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)));
   XT_TRY(dev_alloc(&s->magsum, (size_t)g.N_time * s->NYX));
   XT_HIP(hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)));
   XT_TRY(dev_alloc(&s->stats, 8));
@@ -2214,16 +2341,28 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_TRY(dev_alloc(&s->iter_dev, 1));
   XT_HIP(hipMemcpy(s->iter_dev, &Iter, sizeof(int), hipMemcpyHostToDevice));
   XtIcdLaunch L;
-  L.mode = 2;
-  L.half = 0;
+  L.mode = parity < 0 ? 2 : 3;
+  L.half = parity < 0 ? 0 : parity;
   L.sub_iter = 0;
   L.K = 1;
   L.seed = 0;
   L.count_stats = 1;
-  L.zorder = 1;
+  L.zorder = parity < 0 ? 1 : 0;
   XT_TRY(launch_icd(s, L, 1));
   XT_HIP(hipStreamSynchronize(s->st));
-  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  if (g.pm) {
+    float *tmp = nullptr;
+    XT_TRY(dev_alloc(&tmp, s->nsino));
+    hipLaunchKernelGGL(k_sino_layout, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->e,
+                       tmp, s->nsino, 0);
+    hipError_t e = hipStreamSynchronize(s->st);
+    if (e == hipSuccess)
+      e = hipMemcpy(&E[0][0][0], tmp, s->nsino * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return set_err(XT_E_HIP, "E download: %s", hipGetErrorString(e));
+  } else {
+    XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  }
   XT_TRY(download_object(s, O));
   if (MagUpdate)
     XT_HIP(hipMemcpy(MagUpdate, s->magsum + (size_t)frame * s->NYX, s->NYX * sizeof(float),
@@ -2232,4 +2371,22 @@ extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Rea
   XT_HIP(hipMemcpy(st, s->stats, sizeof st, hipMemcpyDeviceToHost));
   if (total_vox_mag) *total_vox_mag = st[0];
   return XT_OK;
+}
+
+extern "C" int xt_update_lines(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                               int32_t frame, int32_t n_lines, const int32_t *x_list,
+                               const int32_t *y_list, int32_t Iter, Real_t *MagUpdate,
+                               double *total_vox_mag, const uint8_t *Mask) {
+  return update_lines_impl(S, O, T, E, frame, n_lines, x_list, y_list, Iter, MagUpdate,
+                           total_vox_mag, Mask, -1);
+}
+
+extern "C" int xt_update_lines_parity(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E,
+                                      int32_t n_lines, const int32_t *x_list,
+                                      const int32_t *y_list, int32_t Iter, int32_t parity,
+                                      Real_t *MagUpdate, double *total_vox_mag) {
+  if (parity != 0 && parity != 1) return set_err(XT_E_ARG, "parity %d", parity);
+  if (O && O->N_time != 1) return set_err(XT_E_ARG, "one frame (N_time = 1)");
+  return update_lines_impl(S, O, T, E, 0, n_lines, x_list, y_list, Iter, MagUpdate,
+                           total_vox_mag, nullptr, parity);
 }

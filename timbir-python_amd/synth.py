@@ -80,16 +80,7 @@ class ScanConfig:
     def views(self):
         """(angles_deg, times) for every view, in acquisition order."""
         if self.interlaced_K:
-            K, Nt = self.interlaced_K, self.interlaced_N_theta
-            L = Nt // K
-            bits = int(round(math.log2(K)))
-            i = np.arange(self.N_p)
-            sub = (i // L) % K
-            rev = np.zeros_like(sub)
-            for b in range(bits):
-                rev |= ((sub >> b) & 1) << (bits - 1 - b)
-            idx = i * K + rev
-            return idx * (180.0 / Nt), idx / K
+            return interlaced_views(self.interlaced_K, self.interlaced_N_theta, self.N_p)
         ang = np.arange(self.N_p) * (180.0 / self.N_p)
         return ang, np.arange(self.N_p, dtype=np.float64)
 
@@ -160,6 +151,48 @@ def phantom(cfg: ScanConfig) -> np.ndarray:
                 m = (r2 + w) <= 1.0
                 out[f, z][m] = mu
     return out
+
+
+def interlaced_views(K: int, N_theta: int, N_p: int):
+    """Bit-reversed interlaced acquisition order of 4D_Recon_Launcher_for_Real_Data/
+    XT_Interlaced_Angles.py:107-139 (gen_interlaced_views_0_to_Inf): view i has
+    index i*K + bitrev_log2(K)(floor(i / L) mod K), L = N_theta / K; returns
+    (angles in degrees, index * 180 / N_theta, unwrapped; times, index / K)."""
+    L = N_theta // K
+    bits = int(round(math.log2(K)))
+    i = np.arange(N_p)
+    sub = (i // L) % K
+    rev = np.zeros_like(sub)
+    for b in range(bits):
+        rev |= ((sub >> b) & 1) << (bits - 1 - b)
+    idx = (i * K + rev).astype(np.float64)
+    return idx * (180.0 / N_theta), idx / K
+
+
+def clip_list_of_views(views, times, min_time_btw_views: float, rotation_speed: float):
+    """Views the camera cannot take: 4D_Recon_Launcher_for_Real_Data/
+    XT_Interlaced_Angles.py:6-27.  View i (i >= 1) is dropped when it is at most
+    rotation_speed * min_time_btw_views degrees past view i - 1 of the *unclipped*
+    list, else kept.  Returns (kept views, kept times, deleted views, deleted
+    times).  As in the launcher, the first kept time is 0 whatever times[0] is
+    (its output array is zero-initialised and entry 0 is never written)."""
+    views = np.asarray(views, np.float64)
+    times = np.asarray(times, np.float64)
+    min_dtheta = rotation_speed * min_time_btw_views
+    keep = np.ones(len(views), bool)
+    keep[1:] = (views[1:] - views[:-1]) > min_dtheta
+    kept_t = times[keep].copy()
+    if len(kept_t):
+        kept_t[0] = 0.0
+    return views[keep], kept_t, views[~keep], times[~keep]
+
+
+def launcher_views(K: int, N_theta: int, num_cycles: int, min_time_btw_views: float,
+                   rotation_speed: float):
+    """The launcher's view list (4D_Recon_Launcher_for_Real_Data/XT_Initialize.py:
+    73-78): N_theta * num_cycles interlaced views, then clip_list_of_views."""
+    ang, t = interlaced_views(K, N_theta, N_theta * num_cycles)
+    return clip_list_of_views(ang, t, min_time_btw_views, rotation_speed)
 
 
 def write_view_info(path: str, angles_deg, times) -> None:

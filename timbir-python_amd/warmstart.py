@@ -134,6 +134,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--log", default="", help="optional .npz for the loss curve and held-out "
+                    "errors (kept apart from the weights)")
     a = ap.parse_args(argv)
 
     def log(*m):
@@ -159,9 +161,9 @@ def main(argv=None) -> int:
         f"(device fwd {f_ms:.2f} ms + bwd {b_ms:.2f} ms per step)")
     log(f"held-out RMSE to the ICD target, relative to mean attenuation: FBP "
         f"{ev[:, 0].mean():.3%}, network {ev[:, 1].mean():.3%}")
-    np.savez(os.path.splitext(a.out)[0] + "_log.npz", loss=np.array(hist), holdout=ev,
-             pairs=a.pairs, icd_iters=a.icd_iters, epochs=a.epochs, lr=a.lr,
-             train_seconds=t_train)
+    if a.log:
+        np.savez(a.log, loss=np.array(hist), holdout=ev, pairs=a.pairs, icd_iters=a.icd_iters,
+                 epochs=a.epochs, lr=a.lr, train_seconds=t_train)
     return 0
 
 

@@ -31,7 +31,8 @@ EXPORTS = [
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
     "xt_engine_run", "xt_multialloc", "xt_multifree", "xt_detector_response",
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
-    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines", "xt_edge_detect", "xt_probe_footprint_atomics",
+    "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines",
+    "xt_update_lines_parity", "xt_edge_detect", "xt_probe_footprint_atomics",
     "xt_unet_create", "xt_unet_destroy", "xt_unet_param_count", "xt_unet_get_params",
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
@@ -106,7 +107,9 @@ def lib():
         L.xt_variance_update.argtypes = [P, P, P]
         L.xt_vsc_select.argtypes = [P, P, P]
         L.xt_update_lines.argtypes = [P, P, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P,
-                                      C.POINTER(C.c_double)]
+                                      C.POINTER(C.c_double), P]
+        L.xt_update_lines_parity.argtypes = [P, P, P, P, C.c_int32, P, P, C.c_int32, C.c_int32,
+                                             P, C.POINTER(C.c_double)]
         L.xt_unet_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                      C.c_int32, C.POINTER(P)]
         L.xt_unet_destroy.argtypes = [P]
@@ -531,6 +534,22 @@ def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int, mask
     _check(lib().xt_update_lines(*eng.refs(), et.ptr, frame, xs.size, xs.ctypes.data,
                                  ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot),
                                  C.c_void_p(mp)))
+    return mag, tot.value
+
+
+def update_lines_parity(eng: Engine, E: np.ndarray, xs, ys, Iter: int, parity: int):
+    """The production form of a voxel-update pass on the listed lines (one z-block
+    parity, split sinogram, packed lanes; xt_update_lines_parity); returns
+    (MagUpdate [N_y][N_x], total_vox_mag)."""
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    O = eng.O
+    mag = np.zeros((O.N_y, O.N_x), np.float32)
+    tot = C.c_double()
+    et = xt_abi.PtrTree(E)
+    _check(lib().xt_update_lines_parity(*eng.refs(), et.ptr, xs.size, xs.ctypes.data,
+                                        ys.ctypes.data, Iter, parity, mag.ctypes.data,
+                                        C.byref(tot)))
     return mag, tot.value
 
 
