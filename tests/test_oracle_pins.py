@@ -706,3 +706,383 @@ def test_edge_detect_matches_ndimage(scan_root):
         assert not got[0][r > T.radius_obj + 3 * O.delta_xy].any()
     finally:
         o.free(S, O, T)
+
+
+# ---- pins of the oracle's rejection branches, prior cost, temporal term ----
+# The generalized-Huber data term rho(u) (u = e sqrt(w)) is pinned by
+# test_cost_closed_form; the checks below derive what the voxel, offset and
+# variance updates must compute from that pinned cost -- by finite differences
+# and by the minimisation each update performs -- with entries on both sides of
+# the rejection threshold, so a dropped term, a wrong branch or a wrong weight in
+# a rejected entry fails them.
+THETA = dataclasses.replace(TINY, name="theta", p=2.0, alpha=1.0, zinger_T=2.0,
+                            update_offset=0, zero_mean_offset=False, seed=31)
+
+
+def _error_sino(o, S, O, T, vol):
+    """e = y - Ax - d of the object vol (float64), ProjSelect set from it."""
+    o.set_object(O, vol)
+    H = o.detector_response(S, O, T)
+    E = np.zeros((S.N_p, S.N_r, S.N_t))
+    et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+    o.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+    return E, et, H, ht
+
+
+def _cost(o, S, O, T, vol):
+    E, et, _, _ = _error_sino(o, S, O, T, vol)
+    return o.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+
+
+def _theta_object(info, O):
+    rng = np.random.default_rng(8)
+    vol = 0.8 * info["phantom"].astype(np.float64)
+    return vol + rng.uniform(0.0, 2e-4, vol.shape)
+
+
+def test_voxel_theta_from_cost_derivative(oracle64, scan_root):
+    """theta1, theta2 of one voxel update (F/XT_VoxUpdate.c:20-58), read back from
+    the update itself: with p = q = 2 and alpha = 1 the new value is
+    u = (sum b' x_j + theta2 V - theta1) / (sum b' + theta2) with b' = filter *
+    2 / (sigma^2 (c + 1)) known, so three prior strengths give theta1 and theta2
+    (and the residual checks that form).  Then theta1 must be the derivative of
+    the pinned data cost along that voxel (central differences), and the
+    quadratic theta1 D + theta2 D^2 / 2 must majorize the data cost's change,
+    touching at D = 0 -- with rejected entries in the voxel's footprint (T = 2)."""
+    path, info = make_scan(THETA, scan_root)
+    path = oracle_path(path, oracle64)
+    zv, yv, xv = 1, 13, 17
+    vol0 = None
+    rows = []
+    for sigma in (2e-3, 4e-3, 8e-3):
+        cfg = dataclasses.replace(THETA, sigma_s=sigma)
+        S, O, T = oracle64.setup(path, cfg.cli_args())
+        try:
+            oracle64.read_sinogram(path, S, T)
+            if vol0 is None:
+                vol0 = _theta_object(info, O)
+            E, et, H, ht = _error_sino(oracle64, S, O, T, vol0)
+            sel = oracle64.sino_view(S.ProjSelect, S, bool)
+            assert 0.001 < 1.0 - sel.mean() < 0.5  # entries on both sides of the threshold
+            kappa = 2.0 / (T.Sigma_S_Q * (O.C_S + 1.0))
+            filt = np.array(T.Spatial_Filter).reshape(3, 3, 3)
+            V = vol0[0, zv, yv, xv]
+            B = Sx = 0.0
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        if (dz, dy, dx) != (0, 0, 0):
+                            b = filt[dz + 1, dy + 1, dx + 1] * kappa
+                            B += b
+                            Sx += b * vol0[0, zv + dz, yv + dy, xv + dx]
+            xs, ys = np.array([xv], np.int32), np.array([yv], np.int32)
+            mag = np.zeros((O.N_y, O.N_x))
+            mt = xt_abi.PtrTree(mag)
+            zc = C.c_int64(0)
+            oracle64.lib.oracle_update_lines(C.byref(S), C.byref(O), C.byref(T), et.ptr, ht.ptr,
+                                             0, zv, zv, 1, xs.ctypes.data, ys.ctypes.data, 1,
+                                             C.byref(zc), mt.ptr, 0)
+            u = oracle64.get_object(O)[0, zv, yv, xv]
+            rows.append((u - V, 1.0, Sx - u * B))
+        finally:
+            oracle64.free(S, O, T)
+    M = np.array([r[:2] for r in rows])
+    rhs = np.array([r[2] for r in rows])
+    (theta2, theta1), res, _, _ = np.linalg.lstsq(M, rhs, rcond=None)
+    assert np.abs(M @ [theta2, theta1] - rhs).max() <= 1e-9 * np.abs(rhs).max()
+    # the data cost along the voxel, with the prior made negligible (sigma = 1e3)
+    cfg = dataclasses.replace(THETA, sigma_s=1e3)
+    S, O, T = oracle64.setup(path, cfg.cli_args())
+    try:
+        oracle64.read_sinogram(path, S, T)
+
+        def D(delta):
+            v = vol0.copy()
+            v[0, zv, yv, xv] += delta
+            return _cost(oracle64, S, O, T, v)
+        c0 = D(0.0)
+        h = 1e-8
+        d1 = (D(h) - D(-h)) / (2 * h)
+        assert d1 == pytest.approx(theta1, rel=1e-5)
+        assert theta2 > 0
+        for delta in (-3e-3, -1e-3, -2e-4, -3e-5, 3e-5, 2e-4, 1e-3, 3e-3):
+            q = theta1 * delta + 0.5 * theta2 * delta * delta
+            assert q >= D(delta) - c0 - 1e-9 * abs(c0), delta
+        # touching: the gap closes as D^2 (a quadratic of this curvature, not larger)
+        small = 1e-5
+        gap = theta1 * small + 0.5 * theta2 * small * small - (D(small) - c0)
+        assert 0.0 <= gap <= 0.5 * theta2 * small * small + 1e-9 * abs(c0)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def _per_entry_derivatives(o, S, O, T, E, et, h=1e-7):
+    """dC/de for every sinogram entry of the pinned cost (central differences of
+    oracle_compute_cost with one entry moved; ProjSelect held)."""
+    flat = E.reshape(-1)
+    out = np.zeros_like(flat)
+    for k in range(flat.size):
+        e0 = flat[k]
+        flat[k] = e0 + h
+        cp = o.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        flat[k] = e0 - h
+        cm = o.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        flat[k] = e0
+        out[k] = (cp - cm) / (2 * h)
+    return out.reshape(E.shape)
+
+
+OFFS = dataclasses.replace(TINY, name="offs", N_xy=8, N_z=2, N_p=16, zinger_T=2.0, seed=33)
+
+
+@pytest.mark.parametrize("zero_mean", [0, 1])
+def test_offset_update_minimises_majorizer(zero_mean, oracle64, scan_root):
+    """update_Sinogram_Offset (F/XT_ICD_update.c:666-759) with rejected entries:
+    the step of each detector pixel minimises the symmetric quadratic majorizer
+    of the pinned data cost in that pixel's offset -- sum_k wt_k (e_k - s)^2 / 2
+    with wt_k = (dC/de_k) / e_k (the weight whose quadratic has the cost's slope at
+    e_k) -- and with the zero-mean constraint the steps are the constrained
+    minimiser, solved here from its KKT system (so gamma is checked, not only
+    sum d = 0)."""
+    path, info, (S, O, T) = _setup(oracle64, OFFS, scan_root)
+    try:
+        rng = np.random.default_rng(4)
+        shape = (S.N_p, S.N_r, S.N_t)
+        W = rng.uniform(1e3, 2e4, shape)
+        E0 = rng.normal(0.0, 0.012, shape)
+        E0[rng.random(shape) < 0.05] *= 20.0  # beyond the threshold
+        oracle64.sino_view(T.Weight, S)[:] = W
+        sel = np.abs(E0 * np.sqrt(W)) < T.ErrorSinoThresh
+        assert 0.02 < 1.0 - sel.mean() < 0.3
+        oracle64.sino_view(S.ProjSelect, S, bool)[:] = sel
+        oracle64.set_object(O, np.zeros((O.N_time, O.N_z, O.N_y, O.N_x)))
+        d0 = rng.normal(0.0, 0.01, (S.N_r, S.N_t))
+        E = E0.copy()
+        et = xt_abi.PtrTree(E)
+        g = _per_entry_derivatives(oracle64, S, O, T, E, et)
+        wt = g / E0
+        num = (wt * E0).sum(0)
+        den = wt.sum(0)
+        n = num.size
+        if zero_mean:
+            K = np.zeros((n + 1, n + 1))
+            K[:n, :n] = np.diag(den.reshape(-1))
+            K[:n, n] = 1.0
+            K[n, :n] = 1.0
+            sol = np.linalg.solve(K, np.concatenate([num.reshape(-1), [-d0.sum()]]))
+            step = sol[:n].reshape(num.shape)
+        else:
+            step = num / den
+        T.EnforceZeroMeanOffset = zero_mean
+        oracle64.offset_view(S)[:] = d0
+        oracle64.lib.oracle_offset_update(C.byref(S), C.byref(T), et.ptr)
+        d = oracle64.offset_view(S)
+        np.testing.assert_allclose(d - d0, step, rtol=2e-5, atol=1e-12)
+        np.testing.assert_allclose(E, E0 - (d - d0)[None], rtol=1e-12, atol=1e-15)
+        if zero_mean:
+            assert abs(d.sum()) <= 1e-9 * np.abs(d).sum()
+        # ProjSelect follows the new errors
+        assert np.array_equal(oracle64.sino_view(S.ProjSelect, S, bool),
+                              np.abs(E * np.sqrt(W)) < T.ErrorSinoThresh)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def test_variance_fixed_point_minimises_cost(oracle64, scan_root):
+    """update_variance_parameter (F/XT_ICD_update.c:633-664) with rejected entries:
+    repeated, it settles where the pinned MAP cost, as a function of the variance
+    (weights w0 / var, the N/2 log var term), is stationary -- central differences
+    of oracle_compute_cost there vanish and the cost is at a minimum."""
+    path, info, (S, O, T) = _setup(oracle64, OFFS, scan_root)
+    try:
+        rng = np.random.default_rng(6)
+        shape = (S.N_p, S.N_r, S.N_t)
+        w0 = rng.uniform(1e3, 2e4, shape)  # weights for unit variance
+        E = rng.normal(0.0, 0.015, shape)
+        E[rng.random(shape) < 0.05] *= 15.0
+        et = xt_abi.PtrTree(E)
+        oracle64.set_object(O, np.zeros((O.N_time, O.N_z, O.N_y, O.N_x)))
+        var = 1.0
+        oracle64.sino_view(T.Weight, S)[:] = w0 / var
+        oracle64.sino_view(S.ProjSelect, S, bool)[:] = np.abs(E * np.sqrt(w0 / var)) < T.ErrorSinoThresh
+        T.var_est = var
+        for _ in range(200):
+            oracle64.lib.oracle_variance_update(C.byref(S), C.byref(T), et.ptr)
+        var = T.var_est
+        sel = oracle64.sino_view(S.ProjSelect, S, bool).copy()
+        assert 0.01 < 1.0 - sel.mean() < 0.3
+        # the last update left the weights as the reference does: w = w0 / var
+        np.testing.assert_allclose(oracle64.sino_view(T.Weight, S), w0 / var, rtol=1e-9)
+
+        def cost_at(v):
+            oracle64.sino_view(T.Weight, S)[:] = w0 / v
+            oracle64.sino_view(S.ProjSelect, S, bool)[:] = sel
+            T.var_est = v
+            return oracle64.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        h = 1e-6 * var
+        slope = (cost_at(var + h) - cost_at(var - h)) / (2 * h)
+        scale = E.size / var  # the size of either part of the slope
+        assert abs(slope) <= 1e-6 * scale
+        c = cost_at(var)
+        assert c < cost_at(1.05 * var) and c < cost_at(0.95 * var)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def _qggmrf(delta, p, c, sigma_q):
+    """The qGGMRF potential with q = 2 (Thibault et al., Med. Phys. 2007):
+    |d/s|^2 / (c + |d/s|^(2-p)), s = sigma_q^(1/2)."""
+    s = math.sqrt(sigma_q)
+    a = np.abs(delta) / s
+    return a ** 2 / (c + a ** (2.0 - p))
+
+
+PRIOR3D = dataclasses.replace(TINY, name="prior3d", N_xy=5, N_z=3, N_p=12, seed=35)
+PRIOR4D = dataclasses.replace(SMALL4D, name="prior4d", N_xy=5, N_z=3, N_time=2, N_p=32,
+                              interlaced_K=2, interlaced_N_theta=16, seed=37)
+
+
+@pytest.mark.parametrize("cfg", [PRIOR3D, PRIOR4D], ids=["3d", "4d-two-frames"])
+def test_prior_cost_brute_force(cfg, oracle64, scan_root):
+    """The prior half of computeCost (F/XT_ICD_update.c:205-300) on a random object
+    with zero error sinogram (data term 0, log term 0 at unit variance): the sum
+    over unordered pairs of 26-neighbours inside the object of filter * rho_s plus,
+    between consecutive frames, Time_Filter * rho_t -- enumerated here from all 26
+    offsets of every voxel (each pair twice, halved), not the reference's 13
+    causal ones."""
+    path, info, (S, O, T) = _setup(oracle64, cfg, scan_root)
+    try:
+        rng = np.random.default_rng(10)
+        vol = rng.uniform(0.0, 3e-3, (O.N_time, O.N_z, O.N_y, O.N_x))
+        oracle64.set_object(O, vol)
+        E = np.zeros((S.N_p, S.N_r, S.N_t))
+        et = xt_abi.PtrTree(E)
+        oracle64.sino_view(S.ProjSelect, S, bool)[:] = True
+        T.var_est = 1.0
+        got = oracle64.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr)
+        filt = np.array(T.Spatial_Filter).reshape(3, 3, 3)
+        nt, nz, ny, nx = vol.shape
+        want = 0.0
+        for t in range(nt):
+            for z in range(nz):
+                for y in range(ny):
+                    for x in range(nx):
+                        for dz in (-1, 0, 1):
+                            for dy in (-1, 0, 1):
+                                for dx in (-1, 0, 1):
+                                    z2, y2, x2 = z + dz, y + dy, x + dx
+                                    if (dz, dy, dx) == (0, 0, 0) or not (
+                                            0 <= z2 < nz and 0 <= y2 < ny and 0 <= x2 < nx):
+                                        continue
+                                    want += 0.5 * filt[dz + 1, dy + 1, dx + 1] * _qggmrf(
+                                        vol[t, z, y, x] - vol[t, z2, y2, x2], O.MRF_P, O.C_S,
+                                        T.Sigma_S_Q)
+                        if t + 1 < nt:
+                            want += T.Time_Filter[0] * _qggmrf(
+                                vol[t, z, y, x] - vol[t + 1, z, y, x], O.MRF_P, O.C_T,
+                                T.Sigma_T_Q)
+        assert got == pytest.approx(want, rel=1e-12)
+        # a constant shift of the object leaves the prior unchanged
+        oracle64.set_object(O, vol + 1e-3)
+        assert oracle64.lib.oracle_compute_cost(C.byref(S), C.byref(O), C.byref(T), et.ptr) == \
+            pytest.approx(want, rel=1e-9)
+    finally:
+        oracle64.free(S, O, T)
+
+
+def test_init_filter_inverse_distance(oracle64, scan_root):
+    """initFilter (F/XT_Init.c:210-256): the 26 spatial weights are inversely
+    proportional to the neighbour's distance (ratios), the centre weight is 0, the
+    temporal weight equals a face neighbour's (both at distance 1), and the 26
+    spatial plus the two temporal weights sum to mult_xy^3 * delta_Rtime."""
+    for cfg in (TINY, SMALL4D):
+        path, info, (S, O, T) = _setup(oracle64, cfg, scan_root)
+        try:
+            filt = np.array(T.Spatial_Filter).reshape(3, 3, 3)
+            off = np.indices((3, 3, 3)).reshape(3, -1).T - 1
+            dist = np.sqrt((off ** 2).sum(1)).reshape(3, 3, 3)
+            assert filt[1, 1, 1] == 0.0
+            nz = dist > 0
+            prod = filt[nz] * dist[nz]  # constant if weight ~ 1/distance
+            np.testing.assert_allclose(prod, prod[0], rtol=1e-12)
+            assert T.Time_Filter[0] == pytest.approx(filt[1, 1, 2], rel=1e-12)
+            total = filt.sum() + 2 * T.Time_Filter[0]
+            assert total == pytest.approx(O.mult_xy ** 3 * O.delta_Rtime, rel=1e-12)
+        finally:
+            oracle64.free(S, O, T)
+
+
+QUAD4D = dataclasses.replace(QUAD, name="quad4d", N_time=2, N_p=24, interlaced_K=2,
+                             interlaced_N_theta=12, sigma_t=3e-3, c_t=1.0, iterations=600,
+                             seed=4)
+
+
+def test_icd_converges_to_dense_map_solution_4d(oracle64, scan_root):
+    """Two frames, p = q = 2, no rejection, alpha = 1: the MAP problem with the
+    temporal term (F/XT_ICD_update.c:154-169, pairs x(t) - x(t+1) weighted by
+    Time_Filter) is the linear system (A'WA + L_s + L_t) x = A'W y over both
+    frames, each frame seeing only its own views; ICD must reach its dense
+    solution."""
+    path, info = make_scan(QUAD4D, scan_root)
+    path = oracle_path(path, oracle64)
+    S, O, T = oracle64.setup(path, QUAD4D.cli_args())
+    try:
+        assert O.N_time == 2 and T.Time_Filter[0] > 0
+        oracle64.read_sinogram(path, S, T)
+        H = oracle64.detector_response(S, O, T)
+        ht = xt_abi.PtrTree(H)
+        frames, num = oracle64.proj_lists(O)
+        nt, nz, ny, nx = O.N_time, O.N_z, O.N_y, O.N_x
+        nv = nz * ny * nx
+        nsino = S.N_p * S.N_r * S.N_t
+        A = np.zeros((nsino, nt * nv))
+        vals = np.zeros(8)
+        inds = np.zeros(8, np.int32)
+        a = oracle64.AMatrixCol()
+        a.values, a.index = vals.ctypes.data, inds.ctypes.data
+        for t in range(nt):
+            for p in frames[t]:
+                for yy in range(ny):
+                    for xx in range(nx):
+                        oracle64.lib.oracle_amatrix_column(C.byref(S), C.byref(O), ht.ptr,
+                                                           C.byref(a), yy, xx, int(p))
+                        for q in range(a.count):
+                            for z in range(nz):
+                                A[(int(p) * S.N_r + inds[q]) * S.N_t + z,
+                                  t * nv + (z * ny + yy) * nx + xx] = vals[q]
+        y = oracle64.sino_view(S.Projection, S).reshape(-1).copy()
+        w = oracle64.sino_view(T.Weight, S).reshape(-1).copy()
+        filt = np.array(T.Spatial_Filter).reshape(27)
+        ks = 2.0 / (T.Sigma_S_Q * (O.C_S + 1.0))
+        kt = 2.0 / (T.Sigma_T_Q * (O.C_T + 1.0))
+        L = np.zeros((nt * nv, nt * nv))
+        for t in range(nt):
+            for z in range(nz):
+                for yy in range(ny):
+                    for xx in range(nx):
+                        i = t * nv + (z * ny + yy) * nx + xx
+                        for dz in (-1, 0, 1):
+                            for dy in (-1, 0, 1):
+                                for dx in (-1, 0, 1):
+                                    z2, y2, x2 = z + dz, yy + dy, xx + dx
+                                    if (dz, dy, dx) == (0, 0, 0) or not (
+                                            0 <= z2 < nz and 0 <= y2 < ny and 0 <= x2 < nx):
+                                        continue
+                                    k = t * nv + (z2 * ny + y2) * nx + x2
+                                    b = filt[((dz + 1) * 3 + dy + 1) * 3 + dx + 1] * ks
+                                    L[i, i] += b
+                                    L[i, k] -= b
+                        for t2 in (t - 1, t + 1):
+                            if 0 <= t2 < nt:
+                                k = t2 * nv + (z * ny + yy) * nx + xx
+                                b = T.Time_Filter[0] * kt
+                                L[i, i] += b
+                                L[i, k] -= b
+        xs = np.linalg.solve(A.T @ (w[:, None] * A) + L, A.T @ (w * y)).reshape(nt, nz, ny, nx)
+    finally:
+        oracle64.free(S, O, T)
+    oracle64.run(path, QUAD4D.cli_args())
+    x = np.stack([np.fromfile(os.path.join(path, f"object_n0_time_{t}.bin"), np.float64)
+                  .reshape(nz, ny, nx) for t in range(nt)])
+    # the temporal coupling matters on this problem: dropping it moves the solution
+    assert np.abs(x - xs).max() <= 1e-6 * np.abs(xs).max()
