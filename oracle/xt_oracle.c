@@ -1217,7 +1217,15 @@ int32_t oracle_icd_backproject(Sinogram *S, ScannedObject *O, TomoInputs *T, con
   Real_t ****MagUpdateMap = (Real_t ****)alloc4(sizeof(Real_t), O->N_time, nb, O->N_y, O->N_x);
   uint8_t ****Mask = (uint8_t ****)alloc4(sizeof(uint8_t), O->N_time, O->N_y, O->N_x, O->N_z);
 #ifdef _OPENMP
-  if (T->num_threads > 0) omp_set_num_threads(T->num_threads);
+  /* The z-block count follows num_threads (initStructures); the OpenMP team may
+   * be capped separately (ORACLE_OMP_THREADS) so that a host with fewer cores
+   * than num_threads runs the same blocks without oversubscription. */
+  {
+    int32_t nthr = T->num_threads;
+    const char *cap = getenv("ORACLE_OMP_THREADS");
+    if (cap && atoi(cap) > 0 && atoi(cap) < nthr) nthr = atoi(cap);
+    if (nthr > 0) omp_set_num_threads(nthr);
+  }
 #endif
   for (i = 0; i < O->N_time; i++)
     for (j = 0; j < O->N_y; j++)

@@ -1,14 +1,23 @@
 """Writes tests/golden/c1_cpu_icd.npz: the CPU ICD (the oracle's C restatement
 of the reference engine, float build, OpenMP) on the plain 3D workload
 (128x128x8, 256 views over [0,180), 192x8 bins, I0 = 2e4, 0.5% zingers, ring
-offsets; synth.C1) -- the object after 30 iterations and after --iters
-iterations (sampled at 32768 seeded voxel positions), the float build's MAP cost
-after every iteration, and the FP64 MAP cost (f64a build: the float system
-matrix in double arithmetic, e = y - Ax - d recomputed) of both objects with
-their ring offsets.  tests/test_gpu_configs.py runs the GPU engine on the same
-seeded scan and compares.
+offsets; synth.C1):
+  - from the reference's zero start: the object after 30 iterations and after
+    --iters iterations (sampled at 32768 seeded voxel positions), the float
+    build's MAP cost after every iteration;
+  - from a common warm start (the filtered backprojection of oracle/fbp_oracle.py
+    with its ring-offset guess, read with initICD 1 / updateProjOffset 3): the
+    object after --fbp-iters iterations, converged (its own RMSE between 2/3 of
+    the iterations and the last is stored);
+and the FP64 MAP cost (f64a build: the float system matrix in double
+arithmetic, e = y - Ax - d recomputed) of each stored object with its ring
+offsets.  A zero start crawls along the ring-offset/object trade-off for
+hundreds of iterations (8.1% from its own 300-iteration image after 30,
+3.3% after 150), so the converged comparison uses the warm start.
+tests/test_gpu_configs.py runs the GPU engine on the same seeded scan and starts
+and compares.
 
-    python tests/golden/make_c1_reference.py RUN_DIR [--iters 300]
+    python tests/golden/make_c1_reference.py RUN_DIR [--iters 300] [--fbp-iters 150]
 """
 import argparse
 import ctypes as C
@@ -21,9 +30,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import conftest  # noqa: E402,F401  (registers the package)
 import py_oracle  # noqa: E402
 from timbir_python_amd import synth  # noqa: E402
+
+from c1_warm_start import warm_start_args, write_warm_start  # noqa: E402
 
 N_SAMPLE = 32768
 SEED = 20261203
@@ -32,6 +44,7 @@ EARLY = 30
 ap = argparse.ArgumentParser()
 ap.add_argument("run_dir")
 ap.add_argument("--iters", type=int, default=300)
+ap.add_argument("--fbp-iters", type=int, default=150)
 a = ap.parse_args()
 cfg = synth.C1
 d = a.run_dir
@@ -93,6 +106,39 @@ def map_cost64(vol, offset):
 v30, d30 = snap(EARLY)
 vN, dN = snap(a.iters)
 c30, cN = map_cost64(v30, d30), map_cost64(vN, dN)
+
+# the warm-start leg: same scan, FBP start files, initICD 1 / updateProjOffset 3
+dw = d.rstrip("/") + "_fbp"
+fbp_args = warm_start_args(cfg, a.fbp_iters)
+finalw = os.path.join(dw, f"object_n0_time_0_iter_{a.fbp_iters}.bin")
+if not os.path.exists(finalw):
+    os.makedirs(dw, exist_ok=True)
+    for f in ("projection_n0.bin", "weight_n0.bin", "view_info.txt"):
+        with open(os.path.join(d, f), "rb") as src, open(os.path.join(dw, f), "wb") as dst:
+            dst.write(src.read())
+    write_warm_start(dw, cfg)
+    o = py_oracle.Oracle(False)
+    n, arr = py_oracle.argv_array(fbp_args)
+    costs_w = np.full(a.fbp_iters + 1, np.nan)
+    it = C.c_int32()
+    t0 = time.time()
+    rc = o.lib.oracle_engine_run(dw.encode(), n, arr, 1 | 4, costs_w.ctypes.data, C.byref(it))
+    assert rc == 0, rc
+    print("oracle ICD from the FBP start", it.value, "iterations in", time.time() - t0, "s",
+          flush=True)
+    np.save(os.path.join(dw, "costs_f32.npy"), costs_w)
+
+
+def snapw(k):
+    v = np.fromfile(os.path.join(dw, f"object_n0_time_0_iter_{k}.bin"), np.float32)
+    off = np.fromfile(os.path.join(dw, f"proj_offset_n0_iter_{k}.bin"), np.float32)
+    return v, off
+
+
+vW, dW = snapw(a.fbp_iters)
+cW = map_cost64(vW, dW)
+own_w = float(np.sqrt(np.mean((snapw(2 * a.fbp_iters // 3)[0] - vW) ** 2)) / mean_att)
+costs_w = np.load(os.path.join(dw, "costs_f32.npy"))
 # the oracle's own distance to its last iterate, as RMSE / mean attenuation
 own = np.array([np.sqrt(np.mean((snap(k)[0] - vN) ** 2)) / mean_att for k in range(1, a.iters)])
 costs = np.load(os.path.join(d, "costs_f32.npy"))
@@ -101,6 +147,11 @@ np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "c1
                     early_iterations=np.int32(EARLY), final_iterations=np.int32(a.iters),
                     early_cost64=np.float64(c30), final_cost64=np.float64(cN),
                     mean_att=np.float64(mean_att), costs_f32=costs, own_rmse_to_final=own,
+                    fbp_values=vW[idx], fbp_iterations=np.int32(a.fbp_iters),
+                    fbp_cost64=np.float64(cW), fbp_own_rmse=np.float64(own_w),
+                    fbp_costs_f32=costs_w,
                     layout=np.array("[N_z][N_y][N_x] float32 sample, frame 0"))
 print(f"FP64 cost after {EARLY}: {c30!r}, after {a.iters}: {cN!r}; own RMSE to final at "
-      f"{EARLY}: {own[EARLY - 1]:.4%}, at {a.iters // 2}: {own[a.iters // 2 - 1]:.4%}", flush=True)
+      f"{EARLY}: {own[EARLY - 1]:.4%}, at {a.iters // 2}: {own[a.iters // 2 - 1]:.4%}; "
+      f"FBP start: cost after {a.fbp_iters} {cW!r}, own RMSE from 2/3 of the iterations "
+      f"{own_w:.4%}", flush=True)

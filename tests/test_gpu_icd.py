@@ -247,3 +247,4 @@ def test_footprint_atomic_probe():
     assert 1e9 < r < 1e13
     with pytest.raises(xt_hip.XtError):
         xt_hip.probe_footprint_atomics(0)
+

@@ -103,13 +103,17 @@ def test_amatrix_columns_bitexact(cfg, scan_root, oracle32):
         oracle32.free(S, O, T)
 
 
+@pytest.mark.parametrize("atable", [1, 0], ids=["atable", "onthefly"])
 @pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6, NZ32, NZ64], ids=lambda c: c.name)
-def test_forward_project(cfg, scan_root, oracle32, oracle64a):
-    """Ax within 1e-5 relative of the float reference system matrix (FP64 sums).
-    The reference's double build uses a different detector table (its float
-    and double profiles differ by ~3e-3), so the float build is the target."""
-    _print_case("forward_project", cfg)
+def test_forward_project(cfg, atable, scan_root, oracle32, oracle64a):
+    """Ax within 1e-5 relative of the float reference system matrix (FP64 sums),
+    with the columns read from the A-column table (the ICD state's resync) and
+    built on the fly.  The reference's double build uses a different detector
+    table (its float and double profiles differ by ~3e-3), so the float build is
+    the target."""
+    _print_case(f"forward_project atable={atable}", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    xt_hip.set_option(xt_hip.OPT_ATABLE, atable)
     try:
         vol = info["phantom"].astype(np.float32)
         oracle32.set_object(O, vol)
@@ -128,6 +132,7 @@ def test_forward_project(cfg, scan_root, oracle32, oracle64a):
         print(f"  max|Ax_gpu - Ax_ref| / max|Ax| = {rel:.3e} (float-summed oracle: {rel32:.3e})")
         assert rel <= 1e-5
     finally:
+        xt_hip.set_option(xt_hip.OPT_ATABLE, 1)
         eng.close()
         oracle32.free(S, O, T)
 

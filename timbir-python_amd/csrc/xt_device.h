@@ -175,7 +175,16 @@ __global__ void __launch_bounds__(256) k_detector_response(XtGeom g, const float
 // column can reach the chunk; each voxel's exact A column is built once, and
 // its entries times the line's slices are added to the accumulators.
 // mode 0: out64 = Ax.   mode 1: e = y - Ax - d (float).
-template <int ZL, int MA>
+// The A-column table, when the state keeps one (k_build_atable): a candidate
+// voxel's column is then read (first non-zero bin and four values) instead of
+// built; vpos[v] is view v's position in its frame's list.
+struct XtFpTable {
+  const float4 *vals;
+  const int *first;
+  const int *vpos;
+};
+
+template <int ZL, int MA, bool TAB = false>
 __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *__restrict__ x,
                                                          const float *__restrict__ Hr,
                                                          const float *__restrict__ cosv,
@@ -184,7 +193,8 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
                                                          int mode, double *__restrict__ out64,
                                                          const float *__restrict__ y,
                                                          const float *__restrict__ d,
-                                                         float *__restrict__ e) {
+                                                         float *__restrict__ e,
+                                                         XtFpTable tab = {nullptr, nullptr, nullptr}) {
   constexpr int G = 256 / ZL;
   __shared__ double acc[XT_FP_BINS * ZL];
   __shared__ float cval[G][ZL][MA];
@@ -238,7 +248,19 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
           for (int b = 0; b < MA; b++) vals[b] = 0.f;
           if (k <= k1) {
             const int row = by_row ? o : k, col = by_row ? k : o;
-            xt_amatrix_column(g, H, c, s, row, col, &first, vals);
+            if constexpr (TAB) {
+              static_assert(MA == 4, "table columns hold four bins");
+              const size_t ti = ((size_t)f * g.N_y * g.N_x + (size_t)row * g.N_x + col) *
+                                    (size_t)g.max_views + (size_t)tab.vpos[v];
+              const float4 t4 = tab.vals[ti];
+              first = tab.first[ti];
+              vals[0] = t4.x;
+              vals[1] = t4.y;
+              vals[2] = t4.z;
+              vals[3] = t4.w;
+            } else {
+              xt_amatrix_column(g, H, c, s, row, col, &first, vals);
+            }
           }
           cfirst[grp][zl] = first;
 #pragma unroll

@@ -247,6 +247,7 @@ struct xt_icd_state {
   float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *x = nullptr, *Hr = nullptr;
   float *cosv = nullptr, *sinv = nullptr, *viewang = nullptr, *fbp_w = nullptr;
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
+  int *vpos = nullptr;  // each view's position in its frame's list
   int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
   int *nhicd_chunk = nullptr;  // NHICD list compaction: lines per chunk of each map
   uint8_t *line_mask = nullptr;
@@ -284,6 +285,7 @@ static void free_state(xt_icd_state *s) {
   void *bufs[] = {s->e,        s->w,          s->y,        s->d,          s->x,          s->Hr,
                   s->fbp_w,
                   s->cosv,     s->sinv,       s->viewang,  s->views,      s->proj_num,   s->view_frame,
+                  s->vpos,
                   s->lines,    s->nhicd_list, s->nhicd_count, s->nhicd_chunk, s->line_mask, s->nhicd_flag,
                   s->nhicd_blk,
                   s->mag_blk,  s->magsum,     s->filt,     s->nsel,       s->nsel_out,   s->stats,
@@ -311,9 +313,12 @@ static int launch_detector_response(xt_icd_state *s, float beam_width) {
 template <int ZL>
 static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const float *Hr,
                       const float *c, const float *s, const int *vf, int mode, double *out64,
-                      const float *y, const float *d, float *e) {
+                      const float *y, const float *d, float *e, const XtFpTable &tab) {
   dim3 grid((g.N_r + XT_FP_BINS - 1) / XT_FP_BINS, g.N_p);
-  if (g.wide)
+  if (tab.vals && !g.wide)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL, 4, true>), grid, dim3(256), 0, st, g,
+                       x, Hr, c, s, vf, mode, out64, y, d, e, tab);
+  else if (g.wide)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_forward_project<ZL, XT_MAXA_WIDE>), grid, dim3(256), 0, st,
                        g, x, Hr, c, s, vf, mode, out64, y, d, e);
   else
@@ -326,7 +331,8 @@ static void fp_launch(hipStream_t st, const XtGeom &g, const float *x, const flo
 // detector Laplacian.
 static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *x, const float *Hr,
                           const float *c, const float *s, const int *vf, int mode, double *out64,
-                          const float *y, const float *d, float *e, double *pc_tmp = nullptr) {
+                          const float *y, const float *d, float *e, double *pc_tmp = nullptr,
+                          const XtFpTable &tab = {nullptr, nullptr, nullptr}) {
   double *ax = out64;
   int m = mode;
   if (g.pc) {
@@ -335,10 +341,10 @@ static int launch_forward(hipStream_t st, int ZL, const XtGeom &g, const float *
     m = 0;
   }
   switch (ZL) {
-    case 8: fp_launch<8>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
-    case 16: fp_launch<16>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
-    case 32: fp_launch<32>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
-    default: fp_launch<64>(st, g, x, Hr, c, s, vf, m, ax, y, d, e); break;
+    case 8: fp_launch<8>(st, g, x, Hr, c, s, vf, m, ax, y, d, e, tab); break;
+    case 16: fp_launch<16>(st, g, x, Hr, c, s, vf, m, ax, y, d, e, tab); break;
+    case 32: fp_launch<32>(st, g, x, Hr, c, s, vf, m, ax, y, d, e, tab); break;
+    default: fp_launch<64>(st, g, x, Hr, c, s, vf, m, ax, y, d, e, tab); break;
   }
   if (g.pc) {
     const size_t n = (size_t)g.N_p * g.N_r * g.N_t;
@@ -587,7 +593,8 @@ static int check_angles(const Sinogram *S) {
 static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject *O) {
   const XtGeom &g = s->g;
   XT_TRY(check_angles(S));
-  std::vector<int> flat((size_t)g.N_time * g.max_views, 0), num(g.N_time), vf(g.N_p, -1);
+  std::vector<int> flat((size_t)g.N_time * g.max_views, 0), num(g.N_time), vf(g.N_p, -1),
+      vp(g.N_p, 0);
   for (int f = 0; f < g.N_time; f++) {
     num[f] = O->ProjNum[f];
     for (int p = 0; p < O->ProjNum[f]; p++) {
@@ -595,8 +602,11 @@ static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject 
       if (v < 0 || v >= g.N_p) return set_err(XT_E_ARG, "view index %d out of range", v);
       flat[(size_t)f * g.max_views + p] = v;
       vf[v] = f;
+      vp[v] = p;
     }
   }
+  XT_TRY(dev_alloc(&s->vpos, g.N_p));
+  XT_HIP(hipMemcpy(s->vpos, vp.data(), vp.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_TRY(dev_alloc(&s->views, flat.size()));
   XT_TRY(dev_alloc(&s->proj_num, g.N_time));
   XT_TRY(dev_alloc(&s->view_frame, g.N_p));
@@ -695,9 +705,13 @@ static int init_device(int *dev) {
   return XT_OK;
 }
 
+static XtFpTable fp_table(const xt_icd_state *s) {
+  return XtFpTable{s->at_vals, s->at_first, s->vpos};
+}
+
 static int icd_resync(xt_icd_state *s) {
   return launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 1,
-                        nullptr, s->y, s->d, s->e, s->pc_tmp);
+                        nullptr, s->y, s->d, s->e, s->pc_tmp, fp_table(s));
 }
 
 template <int ZL>
@@ -1204,6 +1218,8 @@ static int iterate_one(xt_icd_state *s) {
   const bool nhicd = Iter > 1 && !s->no_nhicd;
   // iterations from the second on are all the same launch sequence: capture it
   // once and replay it (no memset nodes; the iteration number lives on the device)
+  // (per-launch timing needs eager launches: HIP does not report elapsed times
+  // of events recorded inside a graph replay)
   const bool use_graph = g_opt[XT_OPT_GRAPH] != 0 && Iter > 1 && !s->timing;
   if (use_graph) {
     if (s->graph_exec && s->graph_opt_gen != g_opt_gen) drop_graph(s);
@@ -2095,10 +2111,11 @@ extern "C" int xt_forward_project(Sinogram *S, ScannedObject *O, Real_t **H_r, d
   XT_TRY(seam_state(c, S, O, nullptr));
   xt_icd_state *s = c.s;
   XT_TRY(seam_geometry_upload(s, S, O, H_r));
+  XT_TRY(build_atable(s));  // the table the ICD state projects with (XT_OPT_ATABLE)
   double *out = nullptr;
   XT_TRY(dev_alloc(&out, s->nsino));
   int rc = launch_forward(s->st, s->ZL, s->g, s->x, s->Hr, s->cosv, s->sinv, s->view_frame, 0, out,
-                          nullptr, nullptr, nullptr, s->pc_tmp);
+                          nullptr, nullptr, nullptr, s->pc_tmp, fp_table(s));
   if (rc == XT_OK) {
     hipError_t e = hipStreamSynchronize(s->st);
     if (e == hipSuccess) e = hipMemcpy(AX64, out, s->nsino * sizeof(double), hipMemcpyDeviceToHost);

@@ -475,6 +475,17 @@ def forward_project(eng: Engine, H) -> np.ndarray:
     return out
 
 
+def projector(path: str, args):
+    """Callable phantom (float64 [N_time][N_z][N_y][N_x]) -> Ax (float64
+    [N_p][N_r][N_t]) by the GPU forward projector, for the scan whose
+    view_info.txt is in path; the projector synth.generate takes."""
+    def project(vol):
+        with Engine(path, args, read_sinogram=False) as eng:
+            eng.set_volume(vol)
+            return forward_project(eng, detector_response(eng))
+    return project
+
+
 def init_error_sino(eng: Engine, H) -> np.ndarray:
     S = eng.S
     E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
