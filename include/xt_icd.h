@@ -245,6 +245,22 @@ int xt_compute_cost(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_t ***E, d
 /* update_Sinogram_Offset, F/XT_ICD_update.c:666-759. */
 int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E);
 
+/* The ring-offset update of xt_offset_update on device-resident arrays, with no
+ * device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  E and W are device arrays
+ * [N_p][N_r][N_t] (E updated in place), ProjOffset a device array [N_r][N_t]
+ * (updated in place); zinger_T / zinger_delta are ErrorSinoThresh /
+ * ErrorSinoDelta, zero_mean is EnforceZeroMeanOffset.  Results are bit-identical
+ * to xt_offset_update on the same values.  Its FP64 scratch is one 8 MiB array
+ * per GPU, held until the process ends, which limits the detector to about
+ * 131072 pixels (N_r * N_t); larger sizes return XT_E_ARG.  Runs on the null
+ * stream and returns when done.  Calls that overlap on one GPU share the scratch
+ * array and race. */
+int xt_offset_update_noalloc(int32_t N_p, int32_t N_r, int32_t N_t, float zinger_T,
+                             float zinger_delta, int32_t zero_mean, float *E, const float *W,
+                             float *ProjOffset);
+
 /* update_variance_parameter, F/XT_ICD_update.c:633-664. */
 int xt_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E);
 

@@ -2194,6 +2194,73 @@ extern "C" int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
   return seam_download_sel(s, S);
 }
 
+// ---------------------------------------------------------------------------
+// Ring-offset update with no device allocation.  Its FP64 scratch (per
+// view-chunk partial sums, per-pixel sums, per-workgroup sums, gamma) lives in
+// one file-scope device array; each GPU holds its own copy at its own address,
+// looked up once per device with hipGetSymbolAddress and kept here.  Every
+// scratch word a call reads is written earlier in the same call (nothing has to
+// start at zero); calls that overlap on one GPU share the array and race.
+#define XT_NOALLOC_SCRATCH_DOUBLES (1 << 20)  // 8 MiB per GPU
+static __device__ __align__(256) double scratch[XT_NOALLOC_SCRATCH_DOUBLES];
+#define XT_MAX_DEVICES 64
+static double *g_scratch_ptr[XT_MAX_DEVICES];
+
+static int noalloc_scratch(double **out) {
+  int dev = -1;
+  XT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= XT_MAX_DEVICES) return set_err(XT_E_ARG, "device %d", dev);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    XT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return XT_OK;
+}
+
+extern "C" int xt_offset_update_noalloc(int32_t N_p, int32_t N_r, int32_t N_t, float zinger_T,
+                                        float zinger_delta, int32_t zero_mean, float *E,
+                                        const float *W, float *ProjOffset) {
+  if (!E || !W || !ProjOffset) return set_err(XT_E_ARG, "null argument");
+  if (N_p <= 0 || N_r <= 0 || N_t <= 0)
+    return set_err(XT_E_ARG, "empty sinogram %d x %d x %d", N_p, N_r, N_t);
+  if ((size_t)N_p * N_r * N_t * sizeof(float) >= ((size_t)1 << 31))
+    return set_err(XT_E_ARG, "sinogram of %d x %d x %d entries: at most 2^29 - 1", N_p, N_r, N_t);
+  // the view chunking of the allocating form (bit-identical results)
+  const int NRT = N_r * N_t;
+  int nch = (262144 + NRT - 1) / NRT;
+  if (nch > N_p) nch = N_p;
+  const int chunk = (N_p + nch - 1) / nch;
+  nch = (N_p + chunk - 1) / chunk;
+  const int nblk = (NRT + 255) / 256;
+  const size_t need = (size_t)2 * nch * NRT + (size_t)2 * NRT + (size_t)2 * nblk + 8;
+  if (need > XT_NOALLOC_SCRATCH_DOUBLES)
+    return set_err(XT_E_ARG, "%d x %d detector pixels need %zu B of offset scratch; the "
+                   "allocation-free form holds %zu B (at most about 131072 pixels)", N_r, N_t,
+                   need * sizeof(double), (size_t)XT_NOALLOC_SCRATCH_DOUBLES * sizeof(double));
+  double *base = nullptr;
+  XT_TRY(noalloc_scratch(&base));
+  double *part = base, *numden = part + (size_t)2 * nch * NRT, *wg = numden + (size_t)2 * NRT,
+         *gamma = wg + (size_t)2 * nblk;
+  XtGeom g;
+  memset(&g, 0, sizeof g);
+  g.N_p = N_p;
+  g.N_r = N_r;
+  g.N_t = N_t;
+  g.T = zinger_T;
+  g.delta = zinger_delta;
+  const dim3 g2((NRT + 255) / 256, nch);
+  hipLaunchKernelGGL(k_offset_partial, g2, dim3(256), 0, 0, g, E, W, chunk, part);
+  hipLaunchKernelGGL(k_offset_combine, dim3(nblk), dim3(256), 0, 0, g, nch, part, ProjOffset,
+                     numden, wg);
+  hipLaunchKernelGGL(k_offset_gamma, dim3(1), dim3(256), 0, 0, nblk, wg, zero_mean ? 1 : 0, gamma);
+  hipLaunchKernelGGL(k_offset_apply, g2, dim3(256), 0, 0, g, chunk, numden, gamma, ProjOffset, E);
+  XT_HIP(hipGetLastError());
+  XT_HIP(hipStreamSynchronize(0));
+  return XT_OK;
+}
+
 extern "C" int xt_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
   if (!S || !T || !E) return set_err(XT_E_ARG, "null argument");
   ScannedObject O;

@@ -33,6 +33,7 @@ EXPORTS = [
     "xt_amatrix_columns", "xt_forward_project", "xt_init_error_sino", "xt_compute_cost",
     "xt_offset_update", "xt_variance_update", "xt_vsc_select", "xt_update_lines",
     "xt_update_lines_parity", "xt_edge_detect", "xt_probe_footprint_atomics",
+    "xt_offset_update_noalloc",
     "xt_unet_create", "xt_unet_destroy", "xt_unet_param_count", "xt_unet_get_params",
     "xt_unet_set_params", "xt_unet_get_scale", "xt_unet_forward", "xt_unet_loss_grad",
     "xt_unet_adam_step", "xt_unet_train_step", "xt_unet_last_ms", "xt_icd_unet_start",
@@ -108,6 +109,8 @@ def lib():
         L.xt_vsc_select.argtypes = [P, P, P]
         L.xt_update_lines.argtypes = [P, P, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P,
                                       C.POINTER(C.c_double), P]
+        L.xt_offset_update_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                               C.c_float, C.c_int32, P, P, P]
         L.xt_update_lines_parity.argtypes = [P, P, P, P, C.c_int32, P, P, C.c_int32, C.c_int32,
                                              P, C.POINTER(C.c_double)]
         L.xt_unet_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
@@ -546,6 +549,14 @@ def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int, mask
                                  ys.ctypes.data, Iter, mag.ctypes.data, C.byref(tot),
                                  C.c_void_p(mp)))
     return mag, tot.value
+
+
+def offset_update_noalloc(N_p, N_r, N_t, zinger_T, zinger_delta, zero_mean, e_ptr, w_ptr, d_ptr):
+    """xt_offset_update_noalloc on device arrays given by address (E, W: float32
+    [N_p][N_r][N_t]; d: float32 [N_r][N_t])."""
+    _check(lib().xt_offset_update_noalloc(N_p, N_r, N_t, zinger_T, zinger_delta,
+                                          1 if zero_mean else 0, C.c_void_p(e_ptr),
+                                          C.c_void_p(w_ptr), C.c_void_p(d_ptr)))
 
 
 def update_lines_parity(eng: Engine, E: np.ndarray, xs, ys, Iter: int, parity: int):
