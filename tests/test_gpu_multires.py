@@ -100,3 +100,60 @@ def test_two_stage_converged_parity(scan_root, oracle32, oracle64a):
     print(f"  RMSE(gpu-oracle)/mean = {rmse:.4%}, cost gpu {cg:.9g} oracle {cc:.9g}")
     assert rmse <= 0.01
     assert abs(cg - cc) <= 1e-4 * abs(cc)
+
+
+def test_three_stage_coarse_z_converged_parity(scan_root, oracle32, oracle64a):
+    """The launcher's policy with a coarse-z level (XT_Initialize.py:175-190):
+    voxels 4x and 2x wider and two detector rows deep (delta_z 2, z_overlap_num
+    2), then --initICD 3 at full resolution (object upsampled in z, y and x), on
+    the GPU and in the C oracle: converged objects within 1% RMSE and MAP costs
+    within 1e-4."""
+    from test_gpu_icd import map_cost, read_object
+    cfg = dataclasses.replace(SMALL, iterations=120)
+    stages = multires.stage_configs(cfg, 3, z_stages=2)
+    inits = multires.init_icd_of(3, z_stages=2)
+    assert inits == [0, 2, 3] and [c.mult_z for c in stages] == [2, 2, 1]
+    p_gpu, info = make_scan(cfg, scan_root, "mr3-gpu")
+    p_cpu, _ = make_scan(cfg, scan_root, "mr3-cpu")
+    print("[gpu] three-stage engine runs (delta_z 2, 2, 1), GPU and oracle", flush=True)
+    for stage, init, n in zip(stages, inits, (30, 30, cfg.iterations)):
+        args = stage.cli_args(iterations=n, init_icd=init)
+        xt_hip.engine_run(p_gpu, args)
+        oracle32.run(p_cpu, args)
+    xg, xc = read_object(p_gpu, cfg), read_object(p_cpu, cfg)
+    ph = info["phantom"]
+    rmse = np.sqrt(np.mean((xg - xc) ** 2)) / ph[ph > 0].mean()
+    dg = np.fromfile(os.path.join(p_gpu, "proj_offset_n0.bin"), np.float32).reshape(cfg.N_r, cfg.N_t)
+    dc = np.fromfile(os.path.join(p_cpu, "proj_offset_n0.bin"), np.float32).reshape(cfg.N_r, cfg.N_t)
+    cg = map_cost(oracle64a, p_gpu, cfg, xg, dg)
+    cc = map_cost(oracle64a, p_cpu, cfg, xc, dc)
+    print(f"  RMSE(gpu-oracle)/mean = {rmse:.4%}, cost gpu {cg:.9g} oracle {cc:.9g}")
+    assert rmse <= 0.01
+    assert abs(cg - cc) <= 1e-4 * abs(cc)
+
+
+def test_coarse_z_stage_start_beats_zero_start(scan_root):
+    """The stage loop in memory with a coarse-z level: after 3 final-resolution
+    iterations from the upsampled (initICD 3) start the volume is closer to the
+    converged image than 3 iterations from zero."""
+    cfg = dataclasses.replace(SMALL, N_xy=64, seed=14)
+    path, _ = make_scan(cfg, scan_root, "mrz")
+    xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
+    with xt_hip.Engine(path, cfg.cli_args()) as eng:
+        with xt_hip.Icd(eng) as icd:
+            icd.fbp()
+            icd.iterate(60)
+            icd.download()
+        ref = eng.volume()[0].copy()
+    vol, off = multires.coarse_stages(path, cfg, [10, 5], z_stages=2)
+    assert vol.shape == (cfg.N_z // 2, cfg.N_xy // 2, cfg.N_xy // 2)
+    err = {}
+    for start in ("zero", "multires"):
+        with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
+            if start == "multires":
+                multires.start_from_coarse(icd, vol, off, init_icd=3)
+            icd.iterate(3)
+            icd.download()
+            err[start] = float(np.sqrt(np.mean((eng.volume()[0] - ref) ** 2)))
+    print(f"  RMSE to the converged image after 3 iterations: {err}")
+    assert err["multires"] < 0.5 * err["zero"]

@@ -30,6 +30,8 @@ WIDE6 = dataclasses.replace(TINY, name="wide6", N_xy=8, mult_xy=6.0)
 # 16 packed per parity) and the deepest slab a GPU takes (64 slices, 64 lanes)
 NZ32 = dataclasses.replace(TINY, name="nz32", N_xy=32, N_z=32, N_p=48, num_threads=32, seed=21)
 NZ64 = dataclasses.replace(TINY, name="nz64", N_xy=24, N_z=64, N_p=48, num_threads=64, seed=23)
+# slices of two detector rows (delta_z = 2, z_overlap_num 2: a coarse-z multi-resolution stage)
+MZ2 = dataclasses.replace(TINY, name="mz2", N_z=2, mult_z=2, seed=25)
 
 
 def _print_case(name, cfg):
@@ -104,7 +106,8 @@ def test_amatrix_columns_bitexact(cfg, scan_root, oracle32):
 
 
 @pytest.mark.parametrize("atable", [1, 0], ids=["atable", "onthefly"])
-@pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6, NZ32, NZ64], ids=lambda c: c.name)
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D, WIDE3, WIDE6, NZ32, NZ64, MZ2],
+                         ids=lambda c: c.name)
 def test_forward_project(cfg, atable, scan_root, oracle32, oracle64a):
     """Ax within 1e-5 relative of the float reference system matrix (FP64 sums),
     with the columns read from the A-column table (the ICD state's resync) and
@@ -282,9 +285,9 @@ def test_vsc_select_bitexact(cfg, zero_rows, scan_root, oracle32):
 
 @pytest.mark.parametrize("cfg,atable,nt", [(TINY, 1, 0), (TINY, 0, 0), (WIDE3, 0, 0), (WIDE6, 0, 0),
                                             (TINY, 1, 64), (WIDE3, 0, 64), (NZ32, 1, 0),
-                                            (NZ64, 1, 0)],
+                                            (NZ64, 1, 0), (MZ2, 1, 0), (MZ2, 0, 64)],
                          ids=["atable", "onthefly", "wide3", "wide6", "atable-wave", "wide3-wave",
-                              "nz32", "nz64"])
+                              "nz32", "nz64", "mz2", "mz2-onthefly-wave"])
 @pytest.mark.parametrize("Iter", [1, 2])
 def test_update_lines_per_voxel(Iter, cfg, atable, nt, scan_root, oracle32):
     _print_case(f"update_lines Iter={Iter} atable={atable} threads={nt or 256}", cfg)

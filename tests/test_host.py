@@ -39,7 +39,9 @@ def _scalar_fields(st):
     return out
 
 
-@pytest.mark.parametrize("cfg", [TINY, SMALL4D], ids=lambda c: c.name)
+@pytest.mark.parametrize("cfg", [TINY, SMALL4D,
+                                 dataclasses.replace(TINY, name="tiny_mz2", N_z=2, mult_z=2)],
+                         ids=lambda c: c.name)
 def test_engine_setup_matches_oracle(cfg, scan_root, oracle32):
     path, info = make_scan(cfg, scan_root)
     args = cfg.cli_args()

@@ -196,12 +196,14 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
                                                          float *__restrict__ e,
                                                          XtFpTable tab = {nullptr, nullptr, nullptr}) {
   constexpr int G = 256 / ZL;
-  __shared__ double acc[XT_FP_BINS * ZL];
+  // accumulators [bin][slice] plus one spare row that takes the entries of a
+  // column falling outside the chunk (added and never read: no branches)
+  __shared__ double acc[(XT_FP_BINS + 1) * ZL];
   __shared__ float cval[G][ZL][MA];
   __shared__ int cfirst[G][ZL];
   const int v = blockIdx.y, j0 = blockIdx.x * XT_FP_BINS;
   const int tid = threadIdx.x, zl = tid % ZL, grp = tid / ZL;
-  for (int q = tid; q < XT_FP_BINS * ZL; q += 256) acc[q] = 0.0;
+  for (int q = tid; q < (XT_FP_BINS + 1) * ZL; q += 256) acc[q] = 0.0;
   __syncthreads();
   const int f = view_frame[v];
   if (f >= 0) {
@@ -271,16 +273,27 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int nk = min(ZL, k1 - kb + 1);
         if (zl < g.N_z) {
-          for (int m = 0; m < nk; m++) {
-            const int k = kb + m;
-            const int row = by_row ? o : k, col = by_row ? k : o;
-            const double xv = (double)xf[((size_t)row * g.N_x + col) * g.N_z + zl];
-            const int first = cfirst[grp][m];
+          // this lane's slice of the batch's voxels (consecutive along the row or
+          // column o), eight loads in flight ahead of their sums
+          const int stride = by_row ? g.N_z : g.N_x * g.N_z;
+          const float *xb = xf + (by_row ? (o * g.N_x + kb) : (kb * g.N_x + o)) * g.N_z + zl;
+          for (int m0 = 0; m0 < nk; m0 += 8) {
+            float xv[8];
 #pragma unroll
-            for (int b = 0; b < MA; b++) {
-              const float a = cval[grp][m][b];
-              const int jj = first + b - j0;
-              if (a != 0.f && jj >= 0 && jj < XT_FP_BINS) atomicAdd(&acc[jj * ZL + zl], xv * (double)a);
+            for (int u = 0; u < 8; u++) xv[u] = xb[min(m0 + u, nk - 1) * stride];  // in range
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+              const int m = m0 + u;
+              if (m < nk) {
+                const double xd = (double)xv[u];
+                const int first = cfirst[grp][m] - j0;
+#pragma unroll
+                for (int b = 0; b < MA; b++) {
+                  const int jj = first + b;
+                  const int slot = (jj >= 0 && jj < XT_FP_BINS) ? jj : XT_FP_BINS;
+                  atomicAdd(&acc[slot * ZL + zl], xd * (double)cval[grp][m][b]);
+                }
+              }
             }
           }
         }
@@ -291,16 +304,17 @@ __global__ void __launch_bounds__(256) k_forward_project(XtGeom g, const float *
     }
   }
   __syncthreads();
-  // z_overlap_num == 1: slice z is detector row t = z
-  for (int q = tid; q < XT_FP_BINS * ZL; q += 256) {
-    const int jj = q / ZL, z = q - jj * ZL, j = j0 + jj;
-    if (j >= g.N_r || z >= g.N_z) continue;
-    const double a = acc[q];
+  // slice z spans detector rows z * zo .. z * zo + zo - 1, each with the slice's
+  // value (F/XT_ForwardProject.c:8-25)
+  for (int q = tid; q < XT_FP_BINS * g.N_t; q += 256) {
+    const int jj = q / g.N_t, t = q - jj * g.N_t, j = j0 + jj, z = t / g.zo;
+    if (j >= g.N_r) continue;
+    const double a = acc[jj * ZL + z];
     if (mode == 0) {
-      out64[((size_t)v * g.N_r + j) * g.N_t + z] = a;
+      out64[((size_t)v * g.N_r + j) * g.N_t + t] = a;
     } else {
-      const size_t idx = xt_sidx(g, v, j, z);
-      e[idx] = (float)((double)y[idx] - a - (double)d[(size_t)j * g.N_t + z]);
+      const size_t idx = xt_sidx(g, v, j, t);
+      e[idx] = (float)((double)y[idx] - a - (double)d[(size_t)j * g.N_t + t]);
     }
   }
 }
@@ -439,8 +453,17 @@ __global__ void __launch_bounds__(256) k_fbp_backproject(XtGeom g, const float *
     const int i0 = (int)fl;
     const float fr = u - fl;
     if (i0 < -1 || i0 >= g.N_r) continue;
-    const float a = i0 >= 0 ? q[((size_t)v * g.N_r + i0) * g.N_t + z] : 0.f;
-    const float b = i0 + 1 < g.N_r ? q[((size_t)v * g.N_r + i0 + 1) * g.N_t + z] : 0.f;
+    // a slice of zo detector rows takes the mean of its rows
+    float a = 0.f, b = 0.f;
+    for (int r = 0; r < g.zo; r++) {
+      const int t = z * g.zo + r;
+      a += i0 >= 0 ? q[((size_t)v * g.N_r + i0) * g.N_t + t] : 0.f;
+      b += i0 + 1 < g.N_r ? q[((size_t)v * g.N_r + i0 + 1) * g.N_t + t] : 0.f;
+    }
+    if (g.zo > 1) {
+      a /= (float)g.zo;
+      b /= (float)g.zo;
+    }
     acc += wv[v] * (a + fr * (b - a));
   }
   if (!mask[line]) acc = 0.f;
@@ -782,8 +805,10 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
     // ---- 2. theta pass
     const int rot = (int)(xt_hash((uint32_t)c * 0x27D4EB2Fu ^ key ^ (uint32_t)L.half) %
                           (uint32_t)(nv > 0 ? nv : 1));
-    // lane's entry in a row: its slice, or in the split layout its packed position
-    const uint32_t lane_off = (uint32_t)(g.pm ? lz : zs) * sizeof(float);
+    // lane's entry in a row: its slice's first detector row, or in the split layout
+    // its packed position; a slice spans zo rows (z_overlap_num; 1 when packed)
+    const int zo = PACK ? 1 : g.zo;
+    const uint32_t lane_off = (uint32_t)(g.pm ? lz : zs * zo) * sizeof(float);
     const float T2 = g.T * g.T, Td = g.T * g.delta;
     float th1 = 0.f, th2 = 0.f;
     if (lane_on) {
@@ -817,28 +842,31 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
             }
           }
         }
-        float ev[U][NA], wv[U][NA];
+        for (int r = 0; r < zo; r++) {  // the slice's detector rows
+          float ev[U][NA], wv[U][NA];
 #pragma unroll
-        for (int u = 0; u < U; u++)
+          for (int u = 0; u < U; u++)
 #pragma unroll
-          for (int b = 0; b < NA; b++) {
-            const uint32_t o = av[u][b] > 0.f ? off[u] : OOB;
-            ev[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_e, o, b * bin_bytes, 0));
-            wv[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_w, o, b * bin_bytes, 0));
-          }
+            for (int b = 0; b < NA; b++) {
+              const uint32_t o = av[u][b] > 0.f ? off[u] : OOB;
+              const int so = b * bin_bytes + r * (int)sizeof(float);
+              ev[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_e, o, so, 0));
+              wv[u][b] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_w, o, so, 0));
+            }
 #pragma unroll
-        for (int u = 0; u < U; u++)
+          for (int u = 0; u < U; u++)
 #pragma unroll
-          for (int b = 0; b < NA; b++) {
-            const float pe = av[u][b], e1 = ev[u][b], w1 = wv[u][b];
-            // generalized Huber weight: w if |e*sqrt(w)| < T, else T*delta*sqrt(w)/|e|
-            // (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:38-48,76)
-            const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
-            const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
-            const float pw = pe * ws;
-            th2 += pe * pw;
-            th1 -= e1 * pw;
-          }
+            for (int b = 0; b < NA; b++) {
+              const float pe = av[u][b], e1 = ev[u][b], w1 = wv[u][b];
+              // generalized Huber weight: w if |e*sqrt(w)| < T, else T*delta*sqrt(w)/|e|
+              // (F/XT_ICD_update.c:577, F/XT_VoxUpdate.c:38-48,76)
+              const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
+              const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
+              const float pw = pe * ws;
+              th2 += pe * pw;
+              th1 -= e1 * pw;
+            }
+        }
       }
     }
     red1[tid] = th1;
@@ -973,10 +1001,11 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
             av[5] = a2.y;
           }
         }
+        for (int r = 0; r < zo; r++)
 #pragma unroll
-        for (int b = 0; b < NA; b++)
-          __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(-(av[b] * dz), rs_e, av[b] > 0.f ? o : OOB,
-                                                      b * bin_bytes, 0);
+          for (int b = 0; b < NA; b++)
+            __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(-(av[b] * dz), rs_e, av[b] > 0.f ? o : OOB,
+                                                        b * bin_bytes + r * (int)sizeof(float), 0);
       }
     }
     // the next line's loads must see these atomics: wait for all of this

@@ -147,9 +147,10 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
   if ((size_t)S->N_p * (size_t)S->N_r * (size_t)S->N_t * sizeof(float) >= ((size_t)1 << 31))
     return set_err(XT_E_ARG, "sinogram of %d x %d x %d entries: at most 2^29 - 1 entries per GPU",
                    S->N_p, S->N_r, S->N_t);
-  if (S->z_overlap_num != 1 || S->N_t != O->N_z)
-    return set_err(XT_E_ARG, "z_overlap_num=%d (N_t=%d, N_z=%d): only delta_z = 1 is supported",
-                   S->z_overlap_num, S->N_t, O->N_z);
+  // a slice spans z_overlap_num = mult_z detector rows (F/XT_Init.c:617-621)
+  if (S->z_overlap_num < 1 || S->z_overlap_num > 8 || S->N_t != O->N_z * S->z_overlap_num)
+    return set_err(XT_E_ARG, "z_overlap_num=%d with N_t=%d, N_z=%d: need N_t = N_z * "
+                   "z_overlap_num, 1 <= z_overlap_num <= 8", S->z_overlap_num, S->N_t, O->N_z);
   if (O->N_z > 64) return set_err(XT_E_ARG, "N_z=%d > 64 slices per GPU", O->N_z);
   if (2.0 * O->delta_xy / S->delta_r + 2.0 > XT_MAXA_WIDE)
     return set_err(XT_E_ARG, "delta_xy/delta_r = %g too large for one A column (at most 6)",
@@ -158,6 +159,8 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
   g->pc = g_opt[XT_OPT_PHASE_CONTRAST] != 0;
   if (g->pc && g->wide)
     return set_err(XT_E_ARG, "phase contrast needs voxels at most 2 detector bins wide");
+  if (g->pc && S->z_overlap_num != 1)
+    return set_err(XT_E_ARG, "phase contrast needs slices of one detector row (delta_z = 1)");
   for (int i = 0; i < 9; i++) g->lap[i] = S->Lap_Kernel[i / 3][i % 3];
   if (T && T->node_num > 1)
     return set_err(XT_E_ARG, "node_num=%d: this engine runs one slab on one GPU", T->node_num);
@@ -1016,7 +1019,8 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
     }
   s->n_masked = (int)masked.size();
   // split the sinogram by z-block parity when the passes can pack their lanes
-  if (g_opt[XT_OPT_PACK] != 0 && !g.pc && g.nb % 2 == 0 && g.N_z % g.nb == 0 && g.N_t % 2 == 0) {
+  if (g_opt[XT_OPT_PACK] != 0 && !g.pc && g.zo == 1 && g.nb % 2 == 0 && g.N_z % g.nb == 0 &&
+      g.N_t % 2 == 0) {
     g.pm = 1;
     g.bs = g.N_z / g.nb;
     g.H = g.N_t / 2;
@@ -2368,8 +2372,8 @@ static int update_lines_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_
   XT_TRY(seam_state(c, S, O, T));
   xt_icd_state *s = c.s;
   XtGeom &g = s->g;
-  if (parity >= 0 && g_opt[XT_OPT_PACK] != 0 && !g.pc && g.nb % 2 == 0 && g.N_z % g.nb == 0 &&
-      g.N_t % 2 == 0) {
+  if (parity >= 0 && g_opt[XT_OPT_PACK] != 0 && !g.pc && g.zo == 1 && g.nb % 2 == 0 &&
+      g.N_z % g.nb == 0 && g.N_t % 2 == 0) {
     g.pm = 1;
     g.bs = g.N_z / g.nb;
     g.H = g.N_t / 2;

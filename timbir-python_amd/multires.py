@@ -21,26 +21,49 @@ import numpy as np
 from . import synth, xt_hip
 
 
-def stage_configs(cfg: synth.ScanConfig, n_stages: int, first_offsets: bool = False):
-    """The n stage configurations, coarsest first (the last is cfg itself); the
+def z_factors(n_stages: int, z_stages: int = 1):
+    """delta_z of each stage relative to the final one, the launcher's policy
+    (4D_Recon_Launcher_for_Real_Data/XT_Initialize.py:175-179): the z levels
+    2^(z_stages-1) .. 1 on the last z_stages stages, the earlier stages at the
+    coarsest z."""
+    if not 1 <= z_stages <= n_stages:
+        raise ValueError(f"{z_stages} z levels for {n_stages} stages")
+    zs = [2 ** (z_stages - 1 - k) for k in range(z_stages)]
+    return [zs[0]] * (n_stages - z_stages) + zs
+
+
+def init_icd_of(n_stages: int, z_stages: int = 1):
+    """initICD of each stage (XT_Initialize.py:185-188): 0 for the first, 3 where
+    the voxels halve in z as well (object upsampled in z, y and x), else 2."""
+    zf = z_factors(n_stages, z_stages)
+    return [0] + [3 if zf[k] < zf[k - 1] else 2 for k in range(1, n_stages)]
+
+
+def stage_configs(cfg: synth.ScanConfig, n_stages: int, first_offsets: bool = False,
+                  z_stages: int = 1):
+    """The n stage configurations, coarsest first (the last is cfg itself): stage
+    k's voxels are 2^(n-1-k) times the final ones in x and y and z_factors(...)[k]
+    times in z (a slice then spans that many detector rows, z_overlap_num); the
     first stage estimates no ring offsets unless first_offsets (the launcher's
     choice is not to)."""
     out = []
+    zf = z_factors(n_stages, z_stages)
     for k in range(n_stages):
         f = 2 ** (n_stages - 1 - k)
-        if cfg.N_xy % f:
-            raise ValueError(f"N_xy={cfg.N_xy} is not divisible by {f}")
+        if cfg.N_xy % f or cfg.N_z % zf[k]:
+            raise ValueError(f"N_xy={cfg.N_xy}, N_z={cfg.N_z} not divisible by {f}, {zf[k]}")
         c = dataclasses.replace(cfg, N_xy=cfg.N_xy // f, mult_xy=cfg.mult_xy * f,
+                                N_z=cfg.N_z // zf[k], mult_z=cfg.mult_z * zf[k],
                                 update_offset=0 if (k == 0 and not first_offsets)
                                 else cfg.update_offset)
-        if c.N_r != cfg.N_r:
-            raise ValueError("stage sizes must keep the detector width")
+        if c.N_r != cfg.N_r or c.N_t != cfg.N_t:
+            raise ValueError("stage sizes must keep the detector")
         out.append(c)
     return out
 
 
 def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: bool = False,
-                  quick: bool = True):
+                  quick: bool = True, z_stages: int = 1):
     """Runs the stages before the last one (len(iters) of them, iters[k] ICD
     iterations each) on the run folder `path`; returns the last coarse stage's
     object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t].  fbp_first: the
@@ -50,8 +73,12 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     in flight) and the per-iteration FP64 recompute of e (as the reference's
     float build, which never recomputes it): a few iterations of a starting
     guess gain nothing from either (on the 512^2 x 32 slab: 0.66 s instead of
-    0.82 s for 5 + 3 iterations, the same start)."""
-    stages = stage_configs(cfg, len(iters) + 1, first_offsets=fbp_first)[:-1]
+    0.82 s for 5 + 3 iterations, the same start).  z_stages: the number of z
+    levels of the launcher's policy (the returned object then has N_z / 2 slices
+    when the final stage refines z: start_from_coarse with init_icd 3)."""
+    n = len(iters) + 1
+    stages = stage_configs(cfg, n, first_offsets=fbp_first, z_stages=z_stages)[:-1]
+    inits = init_icd_of(n, z_stages)
     vol = off = None
     saved = {o: xt_hip.get_option(o) for o in (xt_hip.OPT_AUTOTUNE, xt_hip.OPT_RESYNC_EVERY)}
     try:
@@ -61,7 +88,7 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
         for k, (c, n) in enumerate(zip(stages, iters)):
             with xt_hip.Engine(path, c.cli_args(iterations=n)) as eng:
                 if vol is not None:
-                    eng.set_volume(xt_hip.upsample_object(2, vol)[None])
+                    eng.set_volume(xt_hip.upsample_object(inits[k], vol)[None])
                     eng.offset()[:] = off
                 with xt_hip.Icd(eng) as icd:
                     if vol is None and fbp_first:
@@ -78,10 +105,12 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     return vol, off
 
 
-def start_from_coarse(icd: xt_hip.Icd, vol: np.ndarray, off: np.ndarray) -> None:
-    """The final stage's start: the coarse object upsampled (initICD 2) and the
-    coarse stage's offsets, uploaded into `icd` (e recomputed)."""
+def start_from_coarse(icd: xt_hip.Icd, vol: np.ndarray, off: np.ndarray,
+                      init_icd: int = 2) -> None:
+    """The final stage's start: the coarse object upsampled (initICD 2, or 3 when
+    the final stage also halves the voxels in z) and the coarse stage's offsets,
+    uploaded into `icd` (e recomputed)."""
     eng = icd.eng
-    eng.set_volume(xt_hip.upsample_object(2, vol)[None])
+    eng.set_volume(xt_hip.upsample_object(init_icd, vol)[None])
     eng.offset()[:] = off
     icd.upload()

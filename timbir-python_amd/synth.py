@@ -9,7 +9,8 @@ reference, the phantom and the true ring offsets.
 
 Geometry convention for every workload here: detector pixel pitch 1 along r and
 t (``length_r = N_r``, ``length_t = N_t``), voxel pitch ``mult_xy`` detector
-pixels in x-y and 1 in z, rotation centre at ``N_r/2``, and an update radius
+pixels in x-y and ``mult_z`` (1 unless a coarse multi-resolution stage says
+otherwise) in z, rotation centre at ``N_r/2``, and an update radius
 covering the whole square, so the A-matrix scaling is the same for every
 variant of the reference (the ``delta_t`` factor of
 Source_Code_4D_Fast/XT_VoxUpdate.c:27 is 1).
@@ -41,6 +42,7 @@ class ScanConfig:
     N_z: int
     N_p: int
     mult_xy: float = 1.5
+    mult_z: int = 1               # detector rows per slice (delta_z; z_overlap_num)
     N_time: int = 1
     I0: float = 2e4
     zinger_frac: float = 0.005
@@ -75,7 +77,7 @@ class ScanConfig:
 
     @property
     def N_t(self) -> int:
-        return self.N_z
+        return self.N_z * self.mult_z
 
     def views(self):
         """(angles_deg, times) for every view, in acquisition order."""
@@ -96,7 +98,7 @@ class ScanConfig:
         r0, rd = self.rtime()
         a = ["--p", repr(self.p), "--sigma_s", sigma_to_hu(self.sigma_s), "--c_s", repr(self.c_s),
              "--sigma_t", sigma_to_hu(self.sigma_t), "--c_t", repr(self.c_t),
-             "--delta_xy", repr(self.mult_xy), "--delta_z", "1",
+             "--delta_xy", repr(self.mult_xy), "--delta_z", str(self.mult_z),
              "--length_r", str(self.N_r), "--length_t", str(self.N_t),
              "--voxel_thresh", repr(self.voxel_thresh), "--cost_thresh", "0",
              "--iter", str(self.iterations if iterations is None else iterations),
