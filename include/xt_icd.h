@@ -107,11 +107,14 @@ enum {
                                 256, 64 (one wave: 4x more lines progress at once, for
                                 short lines and few views per frame), or 0 to let the
                                 create-time autotuner choose (default 0) */
-  XT_OPT_PHASE_CONTRAST = 17 /* 1: phase-contrast tomography (the reference's
+  XT_OPT_PHASE_CONTRAST = 17, /* 1: phase-contrast tomography (the reference's
                                 -DPHASE_CONTRAST_TOMOGRAPHY build): every footprint is the
                                 detector Laplacian of the absorption one (Lap_Kernel), slices
                                 of a line are updated in turn; absorption starts (filtered
                                 backprojection, network) are refused (default 0) */
+  XT_OPT_BH_QUAD_COEF = 18   /* BH_QUAD_COEF of the reference build: the beam-hardening
+                                quadratic coefficient of the counts read with sinobin 3
+                                (default 0) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);
@@ -184,7 +187,12 @@ void xt_icd_batch_destroy(xt_icd_batch *batch);
  * (F/XT_IOMisc.c:224-254).  dir is the run folder (NULL or "" = cwd). */
 int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram *S, ScannedObject *O,
                     TomoInputs *T);
-int xt_engine_read_sinogram(const char *dir, Sinogram *S, TomoInputs *T);
+/* Reads the inputs as genSinogram_fromBin (sinobin 1: projection_n0.bin,
+ * weight_n0.bin) or genSinogram_subsampleCounts (sinobin 3: bright_n0.bin and
+ * weight_n0.bin counts at mult_xy x mult_z times the detector resolution, binned;
+ * beam-hardening coefficient XT_OPT_BH_QUAD_COEF), then divides Weight by var_est
+ * (F/XT_Engine.c:98-123). */
+int xt_engine_read_sinogram(const char *dir, Sinogram *S, ScannedObject *O, TomoInputs *T);
 int xt_engine_write_outputs(const char *dir, Sinogram *S, ScannedObject *O, TomoInputs *T);
 void xt_engine_free(Sinogram *S, ScannedObject *O, TomoInputs *T);
 /* setup + read + ICD_BackProject + write + free, as XT_Engine.c main. */

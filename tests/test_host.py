@@ -215,3 +215,96 @@ def test_oversize_sinogram_refused(n_p, n_r, n_t):
     assert rc == 1  # XT_E_ARG
     assert b"at most 2^29 - 1 entries" in lib.xt_last_error()
     assert not state.value
+
+
+def _counts_run(tmp_path, mxy, mz, n_p=8, nr_full=24, nt_full=8, seed=0):
+    """A run folder with sinobin-3 inputs: bright_n0.bin [nr_full][nt_full] and
+    counts weight_n0.bin [N_p][nr_full][nt_full] at full detector resolution."""
+    rng = np.random.default_rng(seed)
+    d = str(tmp_path)
+    os.makedirs(d, exist_ok=True)
+    ang = np.arange(n_p) * 180.0 / n_p
+    synth.write_view_info(d, ang, np.arange(n_p, dtype=np.float64))
+    bright = rng.uniform(9e3, 1.1e4, (nr_full, nt_full)).astype(np.float32)
+    counts = (bright[None] * np.exp(-rng.uniform(0.0, 2.0, (n_p, nr_full, nt_full)))
+              ).round().astype(np.float32)
+    bright.tofile(os.path.join(d, "bright_n0.bin"))
+    counts.tofile(os.path.join(d, "weight_n0.bin"))
+    args = ["--p", "1.2", "--sigma_s", synth.sigma_to_hu(1e-4), "--c_s", "0.01",
+            "--sigma_t", synth.sigma_to_hu(2e-4), "--c_t", "0.01", "--delta_xy", str(mxy),
+            "--delta_z", str(mz), "--length_r", str(nr_full), "--length_t", str(nt_full),
+            "--voxel_thresh", "0", "--cost_thresh", "0", "--iter", "1",
+            "--rotation_center", str(nr_full / 2.0), "--alpha", "1.5", "--sinobin", "3",
+            "--initICD", "0", "--Rtime0", "0", "--Rtime_delta", "1", "--Rtime_num", "1",
+            "--num_projections", str(n_p), "--N_r", str(nr_full), "--N_t", str(nt_full),
+            "--num_threads", "2", "--radius_obj", str(nr_full), "--updateProjOffset", "0",
+            "--writeTiff", "0", "--zingerT", "4.0", "--zingerDel", "0.1", "--Est_of_Var", "1"]
+    return d, args, bright, counts
+
+
+def test_engine_sinobin3_binning_closed_form(tmp_path):
+    """sinobin 3 (genSinogram_subsampleCounts, F/XT_genSinogram.c:303-370): the
+    detector binned by delta_xy x delta_z (N_r, N_t, rotation centre divided,
+    voxels of one binned pixel, slices of one row), weight = the summed counts,
+    projection = log(summed bright / summed counts) with no beam hardening."""
+    d, args, bright, counts = _counts_run(tmp_path, 2, 2)
+    xt_hip.set_option(xt_hip.OPT_BH_QUAD_COEF, 0.0)
+    with xt_hip.Engine(d, args) as eng:
+        S, O = eng.S, eng.O
+        assert (S.N_r, S.N_t, O.N_x, O.N_z, S.z_overlap_num) == (12, 4, 12, 4, 1)
+        assert O.delta_xy == pytest.approx(S.delta_r) and S.delta_r == pytest.approx(2.0)
+        w = eng.sino("Weight").astype(np.float64)
+        y = eng.sino("Projection").astype(np.float64)
+    cb = counts.astype(np.float64).reshape(8, 12, 2, 4, 2).sum(axis=(2, 4))
+    bb = bright.astype(np.float64).reshape(12, 2, 4, 2).sum(axis=(1, 3))
+    np.testing.assert_allclose(w, cb, rtol=1e-6)
+    np.testing.assert_allclose(y, np.log(bb[None] / cb), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_F), reason="reference sources absent")
+@pytest.mark.parametrize("mxy,mz", [(1, 1), (2, 2), (3, 1)])
+def test_engine_sinobin3_matches_reference(mxy, mz, tmp_path, monkeypatch):
+    """The engine's counts reader against the reference's own compiled
+    genSinogram_subsampleCounts (oracle/_ref, built with BH_QUAD_COEF 0.05) on the
+    same structs and files: projection and weight bit for bit."""
+    import py_oracle
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "gen", f"REF={REF_F}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libxtref_gen_f32.so"))
+    ref.genSinogram_subsampleCounts.argtypes = [C.c_void_p] * 3
+    d, args, bright, counts = _counts_run(tmp_path, mxy, mz, nr_full=24, nt_full=6, seed=mxy)
+    xt_hip.set_option(xt_hip.OPT_BH_QUAD_COEF, 0.05)
+    try:
+        with xt_hip.Engine(d, args) as eng:
+            y_ours = eng.sino("Projection").copy()
+            w_ours = eng.sino("Weight").copy()
+            eng.sino("Projection")[:] = 0
+            eng.sino("Weight")[:] = 0
+            eng.T.debug_file_ptr = py_oracle.devnull_file()
+            monkeypatch.chdir(d)  # the reference reads bright_n0.bin, weight_n0.bin from the cwd
+            assert ref.genSinogram_subsampleCounts(*[C.addressof(x) for x in (eng.S, eng.O, eng.T)]) == 0
+            y_ref = eng.sino("Projection").copy()
+            w_ref = eng.sino("Weight").copy()
+    finally:
+        xt_hip.set_option(xt_hip.OPT_BH_QUAD_COEF, 0.0)
+    assert np.abs(y_ref).max() > 0
+    np.testing.assert_array_equal(w_ours, w_ref)
+    np.testing.assert_array_equal(y_ours, y_ref)
+
+
+def test_engine_sinobin3_coarse_offsets_repeated(tmp_path):
+    """With sinobin 3 a later multi-resolution stage reads the previous stage's
+    ring offsets on its coarser detector (initICD 2: half the bins; 3: half the
+    bins and rows) and repeats each over the pixels it covers (F/XT_Init.c:340-361)."""
+    for init, (pxy, pz) in ((2, (2, 1)), (3, (2, 2))):
+        d, args, _, _ = _counts_run(tmp_path / f"i{init}", 1, 1, nr_full=24, nt_full=8)
+        args = list(args)
+        args[args.index("--initICD") + 1] = str(init)
+        args[args.index("--updateProjOffset") + 1] = "3"
+        coarse = np.arange((24 // pxy) * (8 // pz), dtype=np.float32).reshape(24 // pxy, 8 // pz)
+        coarse.tofile(os.path.join(d, "proj_offset_n0.bin"))
+        np.array([1.0], np.float32).tofile(os.path.join(d, "variance_estimate_n0.bin"))
+        with xt_hip.Engine(d, args, read_sinogram=False) as eng:
+            got = eng.offset().copy()
+        np.testing.assert_array_equal(got, np.repeat(np.repeat(coarse, pxy, 0), pz, 1))

@@ -184,14 +184,28 @@ int parse_args(int argc, char **argv, Sinogram *S, ScannedObject *O, TomoInputs 
       case O_XW: S->x_width = atoi(v); break;
     }
   }
-  if (T->sinobin != 1)
-    return fail(XT_E_ARG, "sinobin=%d: only projection/weight .bin inputs (sinobin 1) are read",
+  if (T->sinobin != 1 && T->sinobin != 3)
+    return fail(XT_E_ARG, "sinobin=%d: the engine reads projection/weight .bin files (1) or "
+                "counts it subsamples itself (3); phantom simulation (2) is not built",
                 (int)T->sinobin);
   if (T->readSino4mHDF)
     return fail(XT_E_ARG, "HDF5 input is read by the ingest module, not by the engine");
-  O->delta_xy = O->mult_xy;  // F/XT_Init.c:617-622
-  O->delta_z = O->mult_z;
-  S->z_overlap_num = (int32_t)O->mult_z;
+  if (T->sinobin == 3) {
+    // F/XT_Init.c:608-616: the detector is binned by mult_xy x mult_z, a voxel is
+    // one binned pixel (the prior keeps its mult_xy scaling), slices are one row
+    if ((int)O->mult_xy < 1 || (int)O->mult_z < 1)
+      return fail(XT_E_ARG, "sinobin 3 needs integer delta_xy, delta_z >= 1");
+    S->total_t_slices = (int32_t)(S->total_t_slices / O->mult_z);
+    S->N_r = (int32_t)(S->N_r / O->mult_xy);
+    O->delta_xy = 1;
+    O->delta_z = 1;
+    T->RotCenter /= O->mult_xy;
+    S->z_overlap_num = 1;
+  } else {
+    O->delta_xy = O->mult_xy;  // F/XT_Init.c:617-622
+    O->delta_z = O->mult_z;
+    S->z_overlap_num = (int32_t)O->mult_z;
+  }
   return XT_OK;
 }
 
@@ -341,8 +355,23 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
     return fail(XT_E_ARG, "out of host memory");
   memset(&S->ProjSelect[0][0][0], 1, (size_t)S->N_p * S->N_r * S->N_t);
   if (T->updateProjOffset == 1 || T->updateProjOffset == 3) {
-    rc = read_floats(dir, "proj_offset_n0.bin", &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
-    if (rc) return rc;
+    if (T->sinobin == 3) {
+      // the previous stage's offsets on its coarser detector, each value repeated
+      // over the pixels it covers (F/XT_Init.c:340-361)
+      const int pxy = (T->initICD == 2 || T->initICD == 3) ? 2 : 1, pz = T->initICD == 3 ? 2 : 1;
+      const int nr = S->N_r / pxy, nt = S->N_t / pz;
+      std::vector<float> c((size_t)nr * nt);
+      rc = read_floats(dir, "proj_offset_n0.bin", c.data(), c.size());
+      if (rc) return rc;
+      for (int i = 0; i < nr; i++)
+        for (int j = 0; j < nt; j++)
+          for (int p = 0; p < pxy; p++)
+            for (int q = 0; q < pz; q++)
+              S->ProjOffset[i * pxy + p][j * pz + q] = c[(size_t)i * nt + j];
+    } else {
+      rc = read_floats(dir, "proj_offset_n0.bin", &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
+      if (rc) return rc;
+    }
   }
   O->x0 = S->R0;
   O->z0 = S->T0;
@@ -384,14 +413,50 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
   return XT_OK;
 }
 
-extern "C" int xt_engine_read_sinogram(const char *dir, Sinogram *S, TomoInputs *T) {
-  size_t n = (size_t)S->N_p * S->N_r * S->N_t;
-  int rc = read_floats(dir, "projection_n0.bin", &S->Projection[0][0][0], n);
+// genSinogram_subsampleCounts, F/XT_genSinogram.c:303-370: bright-field counts
+// [N_r * mult_xy][N_t * mult_z] and counts [N_p][N_r * mult_xy][N_t * mult_z]
+// summed over each binned pixel's mult_xy x mult_z detector pixels (in Real_t, the
+// reference's order); projection = BH t^2 + t with t = log(bright / weight) (the
+// log and the quadratic in double, rounded to Real_t), weight = the summed counts.
+static int read_subsampled_counts(const char *dir, Sinogram *S, ScannedObject *O, TomoInputs *T) {
+  const int mxy = (int)O->mult_xy, mz = (int)O->mult_z;
+  const int NR = S->N_r * mxy, NT = S->N_t * mz;
+  std::vector<float> bright((size_t)NR * NT), weight((size_t)S->N_p * NR * NT);
+  int rc = read_floats(dir, "bright_n0.bin", bright.data(), bright.size());
   if (rc) return rc;
-  rc = read_floats(dir, "weight_n0.bin", &T->Weight[0][0][0], n);
+  rc = read_floats(dir, "weight_n0.bin", weight.data(), weight.size());
+  if (rc) return rc;
+  const double bh = xt_get_option(XT_OPT_BH_QUAD_COEF);
+  for (int i = 0; i < S->N_p; i++)
+    for (int k = 0; k < S->N_r; k++)
+      for (int slice = 0; slice < S->N_t; slice++) {
+        float weight_temp = 0, bright_temp = 0;
+        for (int p = 0; p < mxy; p++)
+          for (int q = 0; q < mz; q++) {
+            weight_temp += weight[((size_t)i * NR + k * mxy + p) * NT + slice * mz + q];
+            bright_temp += bright[(size_t)(k * mxy + p) * NT + slice * mz + q];
+          }
+        const float temp = (float)log((double)(bright_temp / weight_temp));
+        S->Projection[i][k][slice] = (float)(bh * temp * temp + temp);
+        T->Weight[i][k][slice] = weight_temp;
+      }
+  return XT_OK;
+}
+
+extern "C" int xt_engine_read_sinogram(const char *dir, Sinogram *S, ScannedObject *O,
+                                       TomoInputs *T) {
+  if (!S || !O || !T) return fail(XT_E_ARG, "null struct");
+  size_t n = (size_t)S->N_p * S->N_r * S->N_t;
+  int rc;
+  if (T->sinobin == 3) {
+    rc = read_subsampled_counts(dir, S, O, T);
+  } else {
+    rc = read_floats(dir, "projection_n0.bin", &S->Projection[0][0][0], n);
+    if (rc == XT_OK) rc = read_floats(dir, "weight_n0.bin", &T->Weight[0][0][0], n);
+  }
   if (rc) return rc;
   float *w = &T->Weight[0][0][0];
-  for (size_t i = 0; i < n; i++) w[i] /= T->var_est;
+  for (size_t i = 0; i < n; i++) w[i] /= T->var_est;  // F/XT_Engine.c:120-123
   return XT_OK;
 }
 
@@ -446,7 +511,7 @@ extern "C" int xt_engine_run(const char *dir, int argc, char **argv) {
   std::string logp = join(dir, "DEBUG_n0_xt_hip.log");
   FILE *log = fopen(logp.c_str(), "w");
   T.debug_file_ptr = log;
-  if (rc == XT_OK) rc = xt_engine_read_sinogram(dir, &S, &T);
+  if (rc == XT_OK) rc = xt_engine_read_sinogram(dir, &S, &O, &T);
   if (rc == XT_OK && T.reconstruct == 1) {
     std::string saved = g_xt_run_dir;
     g_xt_run_dir = (dir && dir[0]) ? dir : "";

@@ -62,7 +62,7 @@ int xt_report_error(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[18] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0};  // XT_OPT_*
+static double g_opt[19] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0, 0};  // XT_OPT_*
 // bumped whenever an option changes value: a captured iteration graph (which
 // froze the options it was recorded under) is recaptured when it differs
 static uint64_t g_opt_gen = 1;
@@ -72,7 +72,7 @@ extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 17) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 18) return set_err(XT_E_ARG, "unknown option %d", option);
   if (option == XT_OPT_LINE_THREADS && value != 0 && value != 64 && value != 256)
     return set_err(XT_E_ARG, "XT_OPT_LINE_THREADS must be 0, 64 or 256");
   if (g_opt[option] != value) g_opt_gen++;
@@ -80,7 +80,7 @@ extern "C" int xt_set_option(int option, double value) {
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 17) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 18) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {

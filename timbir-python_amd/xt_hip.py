@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
- OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS, OPT_PHASE_CONTRAST) = range(1, 18)
+ OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS, OPT_PHASE_CONTRAST, OPT_BH_QUAD_COEF) = range(1, 19)
 
 # every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
@@ -90,7 +90,7 @@ def lib():
         L.xt_icd_destroy.argtypes = [P]
         L.xt_icd_destroy.restype = None
         L.xt_engine_setup.argtypes = [C.c_char_p, C.c_int, P, P, P, P]
-        L.xt_engine_read_sinogram.argtypes = [C.c_char_p, P, P]
+        L.xt_engine_read_sinogram.argtypes = [C.c_char_p, P, P, P]
         L.xt_engine_write_outputs.argtypes = [C.c_char_p, P, P, P]
         L.xt_engine_free.argtypes = [P, P, P]
         L.xt_engine_free.restype = None
@@ -175,14 +175,15 @@ class Engine:
             lib().xt_engine_free(C.byref(self.S), C.byref(self.O), C.byref(self.T))
             _check(rc)
         if read_sinogram:
-            _check(lib().xt_engine_read_sinogram(path.encode(), C.byref(self.S), C.byref(self.T)))
+            _check(lib().xt_engine_read_sinogram(path.encode(), C.byref(self.S), C.byref(self.O),
+                                                 C.byref(self.T)))
 
     def refs(self):
         return C.byref(self.S), C.byref(self.O), C.byref(self.T)
 
     def read_sinogram(self) -> None:
         _check(lib().xt_engine_read_sinogram(self.path.encode(), C.byref(self.S),
-                                             C.byref(self.T)))
+                                             C.byref(self.O), C.byref(self.T)))
 
     def close(self):
         if self.S is not None:
