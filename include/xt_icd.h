@@ -112,9 +112,13 @@ enum {
                                 detector Laplacian of the absorption one (Lap_Kernel), slices
                                 of a line are updated in turn; absorption starts (filtered
                                 backprojection, network) are refused (default 0) */
-  XT_OPT_BH_QUAD_COEF = 18   /* BH_QUAD_COEF of the reference build: the beam-hardening
+  XT_OPT_BH_QUAD_COEF = 18,  /* BH_QUAD_COEF of the reference build: the beam-hardening
                                 quadratic coefficient of the counts read with sinobin 3
                                 (default 0) */
+  XT_OPT_LINE_GROUP = 19     /* lines a voxel-update workgroup takes at once.  Only 1 is
+                                accepted in this build: groups of 2 (1 x 2 tiles) and 4
+                                (2 x 2 tiles), updated in turn and writing the error
+                                sinogram once per group, are switched off (default 1) */
 };
 int xt_set_option(int option, double value);
 double xt_get_option(int option);

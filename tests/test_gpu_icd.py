@@ -53,22 +53,35 @@ def run_both(cfg, scan_root, oracle32):
     return p_gpu, p_cpu, info
 
 
-@pytest.mark.parametrize("cfg,nhicd_lines,line_threads",
-                         [(dataclasses.replace(SMALL, iterations=150), 1, 0),
-                          (dataclasses.replace(SMALL, iterations=150), 0, 0),
-                          (dataclasses.replace(SMALL, iterations=150), 1, 64),
-                          (dataclasses.replace(SMALL4D, iterations=30), 1, 0),
-                          (dataclasses.replace(SMALL4D, iterations=30), 1, 64)],
+S150 = dataclasses.replace(SMALL, iterations=150)
+S4D30 = dataclasses.replace(SMALL4D, iterations=30)
+GROUPS_OFF = pytest.mark.skip(reason="grouped line updates (XT_OPT_LINE_GROUP 2, 4) are switched "
+                              "off: their first GPU run took a memory fault whose cause is not "
+                              "found")
+
+
+@pytest.mark.parametrize("cfg,nhicd_lines,line_threads,group",
+                         [(S150, 1, 0, 1), (S150, 0, 0, 1), (S150, 1, 64, 1), (S4D30, 1, 0, 1),
+                          (S4D30, 1, 64, 1),
+                          pytest.param(S150, 1, 0, 2, marks=GROUPS_OFF),
+                          pytest.param(S150, 1, 0, 4, marks=GROUPS_OFF),
+                          pytest.param(S150, 0, 0, 4, marks=GROUPS_OFF),
+                          pytest.param(S4D30, 1, 0, 4, marks=GROUPS_OFF),
+                          pytest.param(S4D30, 1, 64, 4, marks=GROUPS_OFF)],
                          ids=["small3d", "small3d-nhicd-per-block", "small3d-one-wave-lines",
-                              "small4d", "small4d-one-wave-lines"])
-def test_converged_parity(cfg, nhicd_lines, line_threads, scan_root, oracle32, oracle64a):
+                              "small4d", "small4d-one-wave-lines", "small3d-groups2",
+                              "small3d-groups4", "small3d-nhicd-per-block-groups4",
+                              "small4d-groups4", "small4d-one-wave-groups4"])
+def test_converged_parity(cfg, nhicd_lines, line_threads, group, scan_root, oracle32, oracle64a):
     xt_hip.set_option(xt_hip.OPT_NHICD_LINES, nhicd_lines)
     xt_hip.set_option(xt_hip.OPT_LINE_THREADS, line_threads)
+    xt_hip.set_option(xt_hip.OPT_LINE_GROUP, group)
     try:
         p_gpu, p_cpu, info = run_both(cfg, scan_root, oracle32)
     finally:
         xt_hip.set_option(xt_hip.OPT_NHICD_LINES, 1)
         xt_hip.set_option(xt_hip.OPT_LINE_THREADS, 0)
+        xt_hip.set_option(xt_hip.OPT_LINE_GROUP, 1)
     xg, xc = read_object(p_gpu, cfg), read_object(p_cpu, cfg)
     ph = info["phantom"]
     mean_att = ph[ph > 0].mean()

@@ -217,6 +217,17 @@ def test_oversize_sinogram_refused(n_p, n_r, n_t):
     assert not state.value
 
 
+def test_line_groups_switched_off():
+    """Grouped line updates (XT_OPT_LINE_GROUP 2 and 4) are switched off in this
+    build: the option refuses them and stays at 1."""
+    lib = xt_hip.lib()
+    for v in (2, 4, 3):
+        assert lib.xt_set_option(xt_hip.OPT_LINE_GROUP, float(v)) == 1
+        assert b"only 1" in lib.xt_last_error()
+    assert lib.xt_get_option(xt_hip.OPT_LINE_GROUP) == 1
+    assert lib.xt_set_option(xt_hip.OPT_LINE_GROUP, 1.0) == 0
+
+
 def _counts_run(tmp_path, mxy, mz, n_p=8, nr_full=24, nt_full=8, seed=0):
     """A run folder with sinobin-3 inputs: bright_n0.bin [nr_full][nt_full] and
     counts weight_n0.bin [N_p][nr_full][nt_full] at full detector resolution."""

@@ -527,6 +527,10 @@ struct XtIcdBuffers {
                              // graph of one iteration replays for every iteration)
   const uint8_t *vmask;      // [N_time][N_y][N_x][N_z] voxels to update (edge-only
                              // updates), or null: every voxel of a listed line
+  const int *groups;         // line groups of the homogeneous passes: [n_groups][gl]
+                             // member lines (-1: none), adjacent lines of one tile
+  int n_groups;
+  int gl;                    // lines per group (1: the single-line kernel)
 };
 
 // ---------------------------------------------------------------------------
@@ -1033,6 +1037,434 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
     atomicAdd(&B.stats[2], n_foot);
     atomicAdd(&B.stats[3], n_vis);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Voxel-line updates in groups of GL lines (homogeneous passes: the lines of a
+// 1 x 2 or 2 x 2 tile of the image, groups taken in a random order; NHICD and
+// listed lines: GL consecutive entries of the list), with the A columns read
+// from the table.  A group's lines are updated one after another exactly as
+// single lines are (same neighbourhood, theta sums and surrogate), except that
+// the error sinogram is written once per group: line l reads e and subtracts,
+// entry by entry, the changes A_m * delta_m of the group's earlier lines m < l
+// -- the value e would hold had they been applied -- and after the last line
+// one atomic per entry of the union of the group's columns applies the sum of
+// all their changes.  Adjacent lines' columns share most bins, so a 2 x 2
+// group issues about half the atomics of four single lines.  Columns of one
+// view that do not fit the union window of SPAN bins get atomics of their own,
+// so any grouping is exact.  Needs slices of one detector row (zo == 1).
+template <int ZL, bool PACK, int NT, int GL>
+__device__ __forceinline__ void icd_groups_body(const XtGeom &g, const XtIcdBuffers &B,
+                                                const XtIcdLaunch &L, const int bx, const int gx,
+                                                const int f) {
+  static_assert(NT == 64 || NT == 256, "workgroup of one or four waves");
+  static_assert(GL == 2 || GL == 4, "groups of two or four lines");
+  constexpr int ZP = PACK ? ZL / 2 : ZL;  // lanes per view group
+  constexpr int G = NT / ZP;              // view groups
+  constexpr int U = 4;                    // views in flight per group and step
+  constexpr int SPAN = 8;                 // bins of a view's union window
+  constexpr uint32_t OOB = 0x80000000u;   // buffer offset that reads 0 / drops the atomic
+  extern __shared__ float4 smem4[];
+  int *vlist_s = (int *)smem4;                                     // [max_views] view index
+  float *red1 = (float *)(vlist_s + g.max_views);                  // [NT]
+  float *red2 = red1 + NT;                                         // [NT]
+  float *xn = red2 + NT;                                           // [11][ZL]
+  float *dxg = xn + 11 * ZL;                                       // [GL][ZL] changes
+  float *th1z = dxg + GL * ZL;                                     // [ZL]
+  float *th2z = th1z + ZL;                                         // [ZL]
+  float *magb = th2z + ZL;                                         // [64]
+  __shared__ int s_cnt_col, s_perf;
+
+  const int tid = threadIdx.x;
+  const int nv = B.proj_num[f];
+  const size_t NYX = (size_t)g.N_y * g.N_x;
+  const int iter = *B.iter;
+  const int blk_sz = g.N_z / g.nb;
+  const int par = (L.mode == 2) ? -1 : ((L.half ^ f) & 1);
+  const int grp = tid / ZP, lz = tid % ZP;
+  int zs = lz;
+  if constexpr (PACK) zs = ((lz / blk_sz) * 2 + par) * blk_sz + lz % blk_sz;
+  auto zblock = [&](int z) {
+    int b = blk_sz > 0 ? z / blk_sz : g.nb - 1;
+    return b >= g.nb ? g.nb - 1 : b;
+  };
+  const bool zs_ok = zs < g.N_z && (par < 0 || (zblock(zs) & 1) == par);
+  for (int p = tid; p < nv; p += NT) vlist_s[p] = B.views[(size_t)f * g.max_views + p];
+
+  // ---- this launch's items: groups of the random order (mode 0); runs of GL
+  // consecutive entries of the workgroup's contiguous share of the NHICD list
+  // (mode 1) or of the line list (modes 2, 3: items bx, bx + gx, ...)
+  int n_items = 0, i0 = 0, istep = 1, r0 = 0, r1 = 0, n_dom = 0;
+  const int *list = B.lines;
+  uint32_t key = 0;
+  if (L.mode == 0) {
+    const int n = B.n_groups;
+    const int per = n / L.K;
+    const int lo = L.sub_iter * per;
+    n_items = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
+    i0 = lo + bx;
+    istep = gx;
+    n_dom = n;
+    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+  } else if (L.mode == 1) {
+    const int hi = B.nhicd_count[f * 2 + par];
+    list = B.nhicd_list + ((size_t)f * 2 + par) * NYX;
+    key = xt_hash(L.seed ^ 0x5BD1E995u ^ xt_hash((uint32_t)iter * 0xC2B2AE35u ^ (uint32_t)f));
+    r0 = (int)((long long)hi * bx / gx);
+    r1 = (int)((long long)hi * (bx + 1) / gx);
+    n_items = (r1 - r0 + GL - 1) / GL;
+  } else {
+    r1 = B.n_lines;
+    n_items = (r1 + GL - 1) / GL;
+    i0 = bx;
+    istep = gx;
+  }
+  if (tid == 0) {
+    s_cnt_col = 0;
+    s_perf = 0;
+  }
+  float vox_mag = 0.f;
+  double n_upd = 0.0, n_foot = 0.0, n_vis = 0.0;
+  const size_t tbl = (size_t)f * NYX * g.max_views;
+  const uint32_t sino_bytes = (uint32_t)((size_t)g.N_p * g.N_r * g.N_t * sizeof(float));
+  const __amdgpu_buffer_rsrc_t rs_e = __builtin_amdgcn_make_buffer_rsrc(B.e, 0, sino_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_w =
+      __builtin_amdgcn_make_buffer_rsrc((void *)B.w, 0, sino_bytes, 0x00020000);
+  const int bin_bytes = (g.pm ? g.H : g.N_t) * (int)sizeof(float);
+  const int plane = g.pm ? par * g.N_p * g.N_r * g.H : 0;
+  const int row_len = g.pm ? g.H : g.N_t;
+  const uint32_t lane_off = (uint32_t)(g.pm ? lz : zs) * sizeof(float);
+  const float T2 = g.T * g.T, Td = g.T * g.delta, dt = g.delta_t;
+  __syncthreads();
+
+  for (int i = i0; i < n_items; i += istep) {
+    int cl[GL];
+    uint32_t gkey;
+    if (L.mode == 0) {
+      const int gi = xt_permute(i, n_dom, key);
+#pragma unroll
+      for (int m = 0; m < GL; m++) cl[m] = B.groups[(size_t)gi * GL + m];
+      gkey = (uint32_t)gi;
+    } else {
+      const int e0 = r0 + i * GL;
+#pragma unroll
+      for (int m = 0; m < GL; m++) cl[m] = e0 + m < r1 ? list[e0 + m] : -1;
+      gkey = (uint32_t)cl[0];
+    }
+    const int rot = (int)(xt_hash(gkey * 0x27D4EB2Fu ^ key ^ (uint32_t)L.half) %
+                          (uint32_t)(nv > 0 ? nv : 1));
+    for (int q = tid; q < GL * ZL; q += NT) dxg[q] = 0.f;
+
+    for (int l = 0; l < GL; l++) {
+      const int c = cl[l];
+      if (c < 0) continue;  // the same for every thread of the workgroup
+      const int yy = c / g.N_x, xx = c - yy * g.N_x;
+      const bool lane_on =
+          zs_ok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblock(zs)) * NYX + c] != 0);
+      const size_t row_l = tbl + (size_t)c * g.max_views;
+      // ---- neighbourhood (the group's earlier lines are already written back)
+      // and the column's entry count
+      int cnt_local = 0;
+      for (int p = tid; p < nv; p += NT) {
+        const float4 t4 = B.at_vals[row_l + p];
+        cnt_local += (t4.x > 0.f) + (t4.y > 0.f) + (t4.z > 0.f) + (t4.w > 0.f);
+      }
+      for (int q = tid; q < 11 * ZL; q += NT) {
+        const int k = q / ZL, z = q - k * ZL;
+        float val = 0.f;
+        if (z < g.N_z) {
+          if (k < 9) {
+            const int y2 = yy + k / 3 - 1, x2 = xx + k % 3 - 1;
+            if (y2 >= 0 && y2 < g.N_y && x2 >= 0 && x2 < g.N_x)
+              val = __hip_atomic_load(&B.x[((size_t)f * NYX + (size_t)y2 * g.N_x + x2) * g.N_z + z],
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            const int f2 = f + (k == 9 ? -1 : 1);
+            if (f2 >= 0 && f2 < g.N_time)
+              val = __hip_atomic_load(&B.x[((size_t)f2 * NYX + c) * g.N_z + z], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        xn[q] = val;
+      }
+      if (cnt_local) atomicAdd(&s_cnt_col, cnt_local);
+      __syncthreads();
+
+      // ---- theta pass over e less the group's earlier lines' changes
+      float dprev[GL];
+#pragma unroll
+      for (int m = 0; m < GL; m++) dprev[m] = (m < l && zs < ZL) ? dxg[m * ZL + zs] * dt : 0.f;
+      bool corr = false;
+#pragma unroll
+      for (int m = 0; m < GL; m++) corr |= dprev[m] != 0.f;
+      float th1 = 0.f, th2 = 0.f;
+      if (lane_on) {
+        for (int q0 = grp; q0 < nv; q0 += U * G) {
+          uint32_t off[U];
+          int first[U], pv[U];
+          float av[U][4], ev[U][4], wv[U][4];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const int q = q0 + u * G;
+            int p = q + rot;
+            if (p >= nv) p -= nv;
+            const bool ok = q < nv;
+            pv[u] = ok ? p : -1;
+            const float4 t4 = ok ? B.at_vals[row_l + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            first[u] = ok ? B.at_first[row_l + p] : 0;
+            av[u][0] = t4.x * dt;
+            av[u][1] = t4.y * dt;
+            av[u][2] = t4.z * dt;
+            av[u][3] = t4.w * dt;
+            off[u] = ok ? (uint32_t)(plane + (vlist_s[p] * g.N_r + first[u]) * row_len) *
+                                  (uint32_t)sizeof(float) + lane_off
+                        : OOB;
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+              const uint32_t o = av[u][b] > 0.f ? off[u] : OOB;
+              ev[u][b] =
+                  __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_e, o, b * bin_bytes, 0));
+              wv[u][b] =
+                  __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_w, o, b * bin_bytes, 0));
+            }
+          if (corr) {
+            // e -= A_m * delta_m of the earlier lines where their columns meet this one's
+#pragma unroll
+            for (int m = 0; m < GL - 1; m++) {
+              if (m < l && cl[m] >= 0) {
+                const size_t rm = tbl + (size_t)cl[m] * g.max_views;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                  if (pv[u] >= 0) {
+                    const float4 t4 = B.at_vals[rm + pv[u]];
+                    const int k0 = first[u] - B.at_first[rm + pv[u]];
+                    const float am[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                      const int k = k0 + b;
+                      float a = 0.f;
+#pragma unroll
+                      for (int kk = 0; kk < 4; kk++) a = (k == kk) ? am[kk] : a;
+                      ev[u][b] -= a * dprev[m];
+                    }
+                  }
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+              const float pe = av[u][b], e1 = ev[u][b], w1 = wv[u][b];
+              // generalized Huber weight as in the single-line kernel
+              const float wr = Td * __builtin_amdgcn_sqrtf(w1) * __builtin_amdgcn_rcpf(fabsf(e1));
+              const float ws = (e1 * e1 * w1 < T2) ? w1 : wr;
+              const float pw = pe * ws;
+              th2 += pe * pw;
+              th1 -= e1 * pw;
+            }
+        }
+      }
+      red1[tid] = th1;
+      red2[tid] = th2;
+      __syncthreads();
+      if (tid < ZP) {
+        float t1 = 0.f, t2 = 0.f;
+        for (int q = 0; q < G; q++) {
+          t1 += red1[q * ZP + tid];
+          t2 += red2[q * ZP + tid];
+        }
+        if (zs < ZL) {
+          th1z[zs] = t1;
+          th2z[zs] = t2;
+        }
+      }
+      if (tid < 64) magb[tid] = 0.f;
+      __syncthreads();
+
+      // ---- surrogate update of the active slices, red-black in z (or ascending)
+      float *xcol = xn + 4 * ZL;
+      const int n_steps = L.zorder ? g.N_z : 2;
+      for (int step = 0; step < n_steps; step++) {
+        const int z = tid;
+        bool upd = false;
+        if (z < g.N_z && (L.zorder ? (z == step) : ((z & 1) == step))) {
+          upd = par < 0 || (zblock(z) & 1) == par;
+          if (upd && L.mode == 1) upd = B.nhicd_blk[((size_t)f * g.nb + zblock(z)) * NYX + c] != 0;
+          if (upd && B.vmask) upd = B.vmask[((size_t)f * NYX + c) * g.N_z + z] != 0;
+        }
+        if (upd) {
+          const float V = xcol[z];
+          float nb[27];
+          bool fl[27];
+          bool any_pos = false;
+#pragma unroll
+          for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+              for (int dx = -1; dx <= 1; dx++) {
+                const int ll = ((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1);
+                const int z2 = z + dz, y2 = yy + dy, x2 = xx + dx;
+                const bool ok = (ll != 13) && z2 >= 0 && z2 < g.N_z && y2 >= 0 && y2 < g.N_y &&
+                                x2 >= 0 && x2 < g.N_x;
+                const float val = ok ? xn[((dy + 1) * 3 + (dx + 1)) * ZL + z2] : 0.f;
+                nb[ll] = val;
+                fl[ll] = ok;
+                any_pos |= ok && (val > 0.f);
+              }
+          const bool tf0 = f > 0, tf1 = f + 1 < g.N_time;
+          const float tn0 = xn[9 * ZL + z], tn1 = xn[10 * ZL + z];
+          any_pos |= (tf0 && tn0 > 0.f) || (tf1 && tn1 > 0.f);
+          const bool skip = (V == 0.f) && (iter > 1) && !any_pos;
+          if (!skip) {
+            float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+            for (int ll = 0; ll < 27; ll++) {
+              if (fl[ll]) {
+                const float bq = g.filt[ll] *
+                                 xt_qggmrf_ratio(V - nb[ll], g.MRF_P, g.C_S, g.Sigma_S_Q, g.Sigma_S_Q_P);
+                s1 += bq * nb[ll];
+                s2 += bq;
+              }
+            }
+            if (tf0) {
+              const float bq =
+                  g.tfilt * xt_qggmrf_ratio(V - tn0, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+              s1 += bq * tn0;
+              s2 += bq;
+            }
+            if (tf1) {
+              const float bq =
+                  g.tfilt * xt_qggmrf_ratio(V - tn1, g.MRF_P, g.C_T, g.Sigma_T_Q, g.Sigma_T_Q_P);
+              s1 += bq * tn1;
+              s2 += bq;
+            }
+            const float T1 = th1z[z], T2z = th2z[z];
+            const float u = (s1 + (T2z * V) - T1) / (s2 + T2z);
+            float nv2 = V + g.alpha * (u - V);
+            if (g.positivity && nv2 <= 0.f) nv2 = 0.f;
+            xcol[z] = nv2;
+            dxg[l * ZL + z] = nv2 - V;
+            atomicAdd(&magb[zblock(z)], fabsf(nv2 - V));
+            vox_mag += fabsf(nv2);
+            atomicAdd(&s_perf, 1);
+          }
+        }
+        __syncthreads();
+      }
+      // ---- write the line back and its update magnitude
+      if (tid < g.N_z)
+        __hip_atomic_store(&B.x[((size_t)f * NYX + c) * g.N_z + tid], xcol[tid], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (tid < g.nb) {
+        const size_t mi = ((size_t)f * g.nb + tid) * NYX + c;
+        if ((par < 0 || (tid & 1) == par) && (L.mode != 1 || B.nhicd_blk[mi] != 0))
+          B.mag_blk[mi] = magb[tid];
+        else
+          magb[tid] = B.mag_blk[mi];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        float sm = 0.f;
+        for (int b = 0; b < g.nb; b++) sm += magb[b];
+        B.magsum[(size_t)f * NYX + c] = sm;
+        n_upd += (double)s_perf;
+        n_foot += (double)s_perf * (double)s_cnt_col;
+        n_vis += 1.0;
+        s_cnt_col = 0;
+        s_perf = 0;
+      }
+      // the next line's neighbourhood loads must see this line's values
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+
+    // ---- e -= sum over the group's lines of A_m * delta_m: per view one atomic
+    // per bin of the window [u0, u0 + SPAN) from the leftmost column, columns
+    // reaching past it on their own
+    float dzm[GL];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < GL; m++) {
+      dzm[m] = (zs < ZL && zs_ok) ? dxg[m * ZL + zs] * dt : 0.f;
+      any |= dzm[m] != 0.f;
+    }
+    if (any) {
+      for (int q = grp; q < nv; q += G) {
+        int p = q + rot;
+        if (p >= nv) p -= nv;
+        float am[GL][4];
+        int fm[GL];
+        int u0 = 0x7fffffff;
+#pragma unroll
+        for (int m = 0; m < GL; m++) {
+          float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
+          fm[m] = 0;
+          if (cl[m] >= 0) {
+            const size_t rm = tbl + (size_t)cl[m] * g.max_views + p;
+            t4 = B.at_vals[rm];
+            fm[m] = B.at_first[rm];
+            u0 = min(u0, fm[m]);
+          }
+          am[m][0] = t4.x;
+          am[m][1] = t4.y;
+          am[m][2] = t4.z;
+          am[m][3] = t4.w;
+        }
+        const uint32_t vrow = (uint32_t)(plane + vlist_s[p] * g.N_r * row_len);
+        const uint32_t o = (vrow + (uint32_t)(u0 * row_len)) * (uint32_t)sizeof(float) + lane_off;
+        bool out[GL];
+#pragma unroll
+        for (int m = 0; m < GL; m++) out[m] = fm[m] + 4 > u0 + SPAN;
+#pragma unroll
+        for (int b = 0; b < SPAN; b++) {
+          float d = 0.f;
+#pragma unroll
+          for (int m = 0; m < GL; m++) {
+            const int k = u0 + b - fm[m];
+            float a = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) a = (k == kk) ? am[m][kk] : a;
+            d += out[m] ? 0.f : a * dzm[m];
+          }
+          __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(-d, rs_e, d != 0.f ? o : OOB,
+                                                        b * bin_bytes, 0);
+        }
+#pragma unroll
+        for (int m = 0; m < GL; m++) {
+          if (out[m]) {
+            const uint32_t om = (vrow + (uint32_t)(fm[m] * row_len)) * (uint32_t)sizeof(float) + lane_off;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+              const float d = am[m][b] * dzm[m];
+              __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(-d, rs_e, d != 0.f ? om : OOB,
+                                                            b * bin_bytes, 0);
+            }
+          }
+        }
+      }
+    }
+    // the next group's loads must see these atomics
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  if (tid < ZL && L.count_stats && vox_mag != 0.f) atomicAdd(&B.stats[0], (double)vox_mag);
+  if (tid == 0 && n_vis != 0.0) {
+    atomicAdd(&B.stats[1], n_upd);
+    atomicAdd(&B.stats[2], n_foot);
+    atomicAdd(&B.stats[3], n_vis);
+  }
+}
+
+template <int ZL, bool PACK, int NT, int GL>
+__global__ void __launch_bounds__(NT) k_icd_groups(XtGeom g, XtIcdBuffers B, XtIcdLaunch L) {
+  icd_groups_body<ZL, PACK, NT, GL>(g, B, L, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
 }
 
 template <int ZL, bool TAB, bool PACK, int MA = XT_MAXA, int NT = 256>

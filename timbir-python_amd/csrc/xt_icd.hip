@@ -62,7 +62,7 @@ int xt_report_error(int code, const char *fmt, ...) {
     if (_rc != XT_OK) return _rc; \
   } while (0)
 
-static double g_opt[19] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0, 0};  // XT_OPT_*
+static double g_opt[20] = {0, 0, 0, 0, 761521, 0, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 0, 0, 0, 1};  // XT_OPT_*
 // bumped whenever an option changes value: a captured iteration graph (which
 // froze the options it was recorded under) is recaptured when it differs
 static uint64_t g_opt_gen = 1;
@@ -72,15 +72,20 @@ extern "C" const char *xt_last_error(void) { return g_err; }
 extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
 
 extern "C" int xt_set_option(int option, double value) {
-  if (option < 1 || option > 18) return set_err(XT_E_ARG, "unknown option %d", option);
+  if (option < 1 || option > 19) return set_err(XT_E_ARG, "unknown option %d", option);
   if (option == XT_OPT_LINE_THREADS && value != 0 && value != 64 && value != 256)
     return set_err(XT_E_ARG, "XT_OPT_LINE_THREADS must be 0, 64 or 256");
+  // groups of 2 or 4 lines (k_icd_groups) are switched off: their first run on the
+  // GPU took a memory fault whose cause is not found (the kernel is memory-safe by
+  // construction and passes the per-voxel checks in a CPU build of this file)
+  if (option == XT_OPT_LINE_GROUP && value != 1)
+    return set_err(XT_E_ARG, "XT_OPT_LINE_GROUP: only 1 (single lines) is enabled in this build");
   if (g_opt[option] != value) g_opt_gen++;
   g_opt[option] = value;
   return XT_OK;
 }
 extern "C" double xt_get_option(int option) {
-  return (option >= 1 && option <= 18) ? g_opt[option] : NAN;
+  return (option >= 1 && option <= 19) ? g_opt[option] : NAN;
 }
 
 static std::string run_path(const char *name) {
@@ -252,6 +257,10 @@ struct xt_icd_state {
   int *views = nullptr, *proj_num = nullptr, *view_frame = nullptr;
   int *vpos = nullptr;  // each view's position in its frame's list
   int *lines = nullptr, *nhicd_list = nullptr, *nhicd_count = nullptr;
+  // the homogeneous passes' line groups: 1 x 2 tiles (groups2) and 2 x 2 tiles
+  // (groups4) of the line list, members first and -1 after them
+  int *groups2 = nullptr, *groups4 = nullptr;
+  int n_groups2 = 0, n_groups4 = 0;
   int *nhicd_chunk = nullptr;  // NHICD list compaction: lines per chunk of each map
   uint8_t *line_mask = nullptr;
   uint8_t *nhicd_flag = nullptr, *nhicd_blk = nullptr;
@@ -296,7 +305,7 @@ static void free_state(xt_icd_state *s) {
                   s->magpar,
                   s->red_wg,   s->scal,       s->off_part, s->off_numden, s->off_wg,
                   s->at_vals,  s->at_first,   s->iter_dev, s->pc_tmp, s->vmask, s->vsc_st,
-                  s->vsc_hist};
+                  s->vsc_hist, s->groups2, s->groups4};
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *b : bufs)
@@ -370,7 +379,23 @@ static void icd_launch_nt(hipStream_t st, const XtGeom &g, const XtIcdBuffers &B
                           const XtIcdLaunch &L, int gx, size_t lds) {
   const dim3 grid(gx, g.N_time), block(NT);
   const bool pack = icd_packable(g, ZL, L);
-  if (g.wide) {  // never with the table (it holds four bins per column)
+  if (B.gl == 2 || B.gl == 4) {  // groups of lines, from the table
+    if (B.gl == 2) {
+      if (pack)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_groups<ZL, true, NT, 2>), grid, block, lds, st, g,
+                           B, L);
+      else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_groups<ZL, false, NT, 2>), grid, block, lds, st, g,
+                           B, L);
+    } else {
+      if (pack)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_groups<ZL, true, NT, 4>), grid, block, lds, st, g,
+                           B, L);
+      else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_groups<ZL, false, NT, 4>), grid, block, lds, st, g,
+                           B, L);
+    }
+  } else if (g.wide) {  // never with the table (it holds four bins per column)
     if (pack)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_icd_lines<ZL, false, true, XT_MAXA_WIDE, NT>), grid,
                          block, lds, st, g, B, L);
@@ -426,13 +451,41 @@ static XtIcdBuffers buffers_of(xt_icd_state *s) {
   B.at_first = s->at_first;
   B.iter = s->iter_dev;
   B.vmask = s->vmask;
+  B.groups = nullptr;
+  B.n_groups = 0;
+  B.gl = 1;
   return B;
 }
 
 // One voxel-update launch over a range of lines; gx workgroups per frame.
+// Lines per workgroup item of the voxel-update launches (XT_OPT_LINE_GROUP): 2
+// or 4 with the A-column table and slices of one detector row, else 1.
+static int line_group(const xt_icd_state *s) {
+  const int gl = (int)g_opt[XT_OPT_LINE_GROUP];
+  if ((gl != 2 && gl != 4) || !s->at_vals || s->g.wide || s->g.pc || s->g.zo != 1) return 1;
+  return gl;
+}
+
+static size_t group_lds_bytes(const XtGeom &g, int ZL, int gl, int nt) {
+  return (size_t)g.max_views * sizeof(int) + 2 * (size_t)nt * sizeof(float) +
+         (size_t)(13 + gl) * ZL * sizeof(float) + 64 * sizeof(float);
+}
+
 static int launch_icd(xt_icd_state *s, const XtIcdLaunch &L, int gx) {
   XtIcdBuffers B = buffers_of(s);
   size_t lds = icd_lds_bytes(s->g, s->ZL, B.at_vals != nullptr, s->nt);
+  const int gl = line_group(s);
+  if (gl > 1) {
+    B.gl = gl;
+    B.groups = gl == 2 ? s->groups2 : s->groups4;
+    B.n_groups = gl == 2 ? s->n_groups2 : s->n_groups4;
+    lds = group_lds_bytes(s->g, s->ZL, gl, s->nt);
+    if (L.mode == 0) {  // no more workgroups than the sub-iteration has groups
+      const int per = B.n_groups / L.K;
+      const int share = (L.sub_iter == L.K - 1) ? B.n_groups - L.sub_iter * per : per;
+      if (gx > share) gx = share;
+    }
+  }
   if (gx < 1) gx = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing) {
@@ -561,6 +614,37 @@ static int vsc_select(xt_icd_state *s) {
 // lines read each other's stale error sinogram; a CPU simulation of that
 // coupling kept the reference's trajectory up to ~6% of the lines at once
 // and diverged at 25%), or XT_OPT_LINES when set.
+// The groups of the homogeneous passes (XT_OPT_LINE_GROUP 2 and 4): the listed
+// lines of each 1 x 2 and of each 2 x 2 tile of the image, tiles in raster order.
+static int build_groups(xt_icd_state *s, const std::vector<int> &lines) {
+  const int NX = s->g.N_x, NY = s->g.N_y;
+  std::vector<uint8_t> in(s->NYX, 0);
+  for (int c : lines) in[c] = 1;
+  for (int gl : {2, 4}) {
+    const int th = gl == 4 ? 2 : 1;  // tile rows
+    std::vector<int> grp;
+    for (int ty = 0; ty < NY; ty += th)
+      for (int tx = 0; tx < NX; tx += 2) {
+        int mem[4], n = 0;
+        for (int dy = 0; dy < th; dy++)
+          for (int dx = 0; dx < 2; dx++) {
+            const int y = ty + dy, x = tx + dx;
+            if (y < NY && x < NX && in[(size_t)y * NX + x]) mem[n++] = y * NX + x;
+          }
+        if (n == 0) continue;
+        for (int m = 0; m < gl; m++) grp.push_back(m < n ? mem[m] : -1);
+      }
+    int *&dst = gl == 2 ? s->groups2 : s->groups4;
+    if (dst) (void)hipFree(dst);
+    dst = nullptr;
+    XT_TRY(dev_alloc(&dst, grp.empty() ? 1 : grp.size()));
+    if (!grp.empty())
+      XT_HIP(hipMemcpy(dst, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
+    (gl == 2 ? s->n_groups2 : s->n_groups4) = (int)(grp.size() / gl);
+  }
+  return XT_OK;
+}
+
 static int setup_lines(xt_icd_state *s, const std::vector<int> &masked) {
   int conc = (int)g_opt[XT_OPT_LINES];
   if (conc <= 0) {
@@ -582,7 +666,7 @@ static int setup_lines(xt_icd_state *s, const std::vector<int> &masked) {
     XT_HIP(hipMemcpy(s->lines, masked.data(), masked.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->line_mask, in.data(), s->NYX, hipMemcpyHostToDevice));
   XT_HIP(hipMemset(s->nhicd_count, 0, s->g.N_time * 2 * sizeof(int)));
-  return XT_OK;
+  return build_groups(s, masked);
 }
 
 // the detector-table kernel reduces angles by repeated subtraction of pi/2
@@ -969,6 +1053,7 @@ static int select_edges(xt_icd_state *s) {
   s->n_masked = (int)lines.size();
   if (!lines.empty())
     XT_HIP(hipMemcpy(s->lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
+  XT_TRY(build_groups(s, lines));
   XT_HIP(hipMemcpy(s->nsel, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->nsel_par, np.data(), np.size() * sizeof(int), hipMemcpyHostToDevice));
   drop_graph(s);

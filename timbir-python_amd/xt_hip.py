@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(HERE, "libxt_hip.so")
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
- OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS, OPT_PHASE_CONTRAST, OPT_BH_QUAD_COEF) = range(1, 19)
+ OPT_AUTOTUNE, OPT_BATCH_FUSE, OPT_LINE_THREADS, OPT_PHASE_CONTRAST, OPT_BH_QUAD_COEF,
+ OPT_LINE_GROUP) = range(1, 20)
 
 # every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
