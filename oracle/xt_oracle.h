@@ -19,6 +19,9 @@
  *   oracle_forward_project       vs the reference's forward_project_voxel (oracle/_ref)
  *                                summed over all voxels, and dense-matrix brute force
  *   oracle_vsc_select            bit-exact vs the reference's XT_NHICD.c (oracle/_ref)
+ *   oracle_random_select         each (frame, block)'s masked lines selected once; N_y*N_x
+ *                                draws of the pinned RNG; first and first-half positions
+ *                                uniform over 4000 seeds (three planted mistakes fail)
  *   oracle_parse_view_info       the reference's view_info.txt fixtures
  *   oracle_interlaced_views      golden log Source_Code_4D_Fast/info.txt:1-175
  *   oracle_counts_to_sinogram    closed forms: flat fields give 0, Beer-Lambert ratios
@@ -27,9 +30,19 @@
  *   oracle_upsample_object       closed forms: constants and linear ramps reproduced
  *                                exactly (tests/test_oracle_pins.py)
  *   voxel update / surrogate     closed forms: quadratic prior + no zingers reduces ICD to
- *                                Gauss-Seidel on (A'WA+H)x = A'W(y-d), solved densely;
- *                                qGGMRF derivative vs finite differences of the value
- *   offset / variance / cost     closed forms on small inputs (weighted means, sums)
+ *                                Gauss-Seidel on (A'WA+H)x = A'W(y-d), solved densely (one
+ *                                frame, and two frames with the temporal term);
+ *                                qGGMRF derivative vs finite differences of the value;
+ *                                theta1/theta2 read back from updates = the data cost's
+ *                                derivative along the voxel (central differences) and
+ *                                its majorizer, with rejected entries
+ *   offset update                each pixel's step minimises the symmetric IRLS majorizer
+ *                                of the data cost; zero-mean steps = the KKT solution
+ *   variance                     closed form mean(e^2 w); its fixed point is the cost's
+ *                                stationary point in sigma^2
+ *   cost                         closed form on a zero object; prior half = brute-force
+ *                                sum over unordered 26-neighbour pairs (3D, two frames)
+ *   prior filter weights         inverse-distance ratios, centre 0, sum mult_xy^3 dRtime
  *   full ICD driver              converges to the dense MAP solution (quadratic case);
  *                                the reference's own ICD driver (XT_ICD_update.c) cannot
  *                                be built here (it needs mpi.h), so end-to-end trajectory

@@ -1086,3 +1086,60 @@ def test_icd_converges_to_dense_map_solution_4d(oracle64, scan_root):
                   .reshape(nz, ny, nx) for t in range(nt)])
     # the temporal coupling matters on this problem: dropping it moves the solution
     assert np.abs(x - xs).max() <= 1e-6 * np.abs(xs).max()
+
+
+# -- random line selection ----------------------------------------------------
+SEL = dataclasses.replace(TINY, name="sel", N_xy=8, N_z=4, N_p=12, num_threads=2, seed=31)
+
+
+def test_random_select_is_a_uniform_permutation(oracle32, scan_root):
+    """randomly_select_x_y (F/XT_ICD_update.c:599-631) on a random voxel mask, per
+    (frame, z-block): every line holding a masked voxel of the block is selected
+    exactly once and no other line is; exactly N_y*N_x numbers of the pinned RNG
+    are drawn per (frame, block); and over many seeds every selected line comes
+    first, and lands in the first half of the order, as often as a uniform shuffle
+    makes it (the raster-last line included, which a draw scaled by one less than
+    the remaining count could never pick first)."""
+    path, info, (S, O, T) = _setup(oracle32, SEL, scan_root)
+    try:
+        nb, nxy, nz = T.num_z_blocks, O.N_y * O.N_x, O.N_z
+        bs = nz // nb
+        rng = np.random.default_rng(8)
+        mask = (rng.random((O.N_time, O.N_y, O.N_x, nz)) < 0.3).astype(np.uint8)
+        mask[:, 2] = 0  # a row of lines with no masked voxel
+        mask[:, -1, -1, :] = 1  # the raster-last line in every block
+        mt = xt_abi.PtrTree(mask)
+        sel = [mask[0, :, :, b * bs:(b + 1) * bs].any(axis=2).reshape(-1) for b in range(nb)]
+        first = np.zeros((nb, nxy))
+        firsthalf = np.zeros((nb, nxy))
+        n_seeds = 4000
+        for seed in range(1, n_seeds + 1):
+            oracle32.lib.oracle_srandom2(seed)
+            oracle32.lib.oracle_random_select(C.byref(O), C.byref(T), mt.ptr)
+            num = py_oracle.deref_data(T.UpdateSelectNum, 1, C.c_int32, O.N_time * nb).copy()
+            xs = py_oracle.deref_data(T.x_rand_select, 2, C.c_int32, O.N_time * nb * nxy)
+            ys = py_oracle.deref_data(T.y_rand_select, 2, C.c_int32, O.N_time * nb * nxy)
+            for k in range(O.N_time * nb):
+                f, b = divmod(k, nb)
+                got = (ys.reshape(-1, nxy)[k, :num[k]] * O.N_x + xs.reshape(-1, nxy)[k, :num[k]])
+                if seed <= 50:
+                    want = np.nonzero(mask[f, :, :, b * bs:(b + 1) * bs].any(axis=2).reshape(-1))[0]
+                    assert len(set(got.tolist())) == num[k]
+                    assert sorted(got.tolist()) == want.tolist()
+                if f == 0:
+                    first[b, got[0]] += 1
+                    firsthalf[b, got[: num[k] // 2]] += 1
+            if seed <= 50:
+                nxt = oracle32.lib.oracle_random2()
+                draws = np.zeros(O.N_time * nb * nxy + 1)
+                oracle32.lib.oracle_random2_seq(seed, draws.size, draws.ctypes.data)
+                assert nxt == draws[-1]  # N_y*N_x draws per (frame, block)
+        for b in range(nb):
+            n = int(sel[b].sum())
+            for counts, p in ((first[b], 1.0 / n), (firsthalf[b], (n // 2) / n)):
+                freq = counts[sel[b]] / n_seeds
+                sd = math.sqrt(p * (1 - p) / n_seeds)
+                assert np.abs(freq - p).max() < 4.5 * sd, (b, p, freq.min(), freq.max())
+            assert first[b, ~sel[b]].sum() == 0
+    finally:
+        oracle32.free(S, O, T)
