@@ -12,20 +12,28 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, SMALL4D, TINY, make_scan
-from timbir_python_amd import synth, xt_hip
+from conftest import ROOT, SMALL4D, TINY, as_double, make_scan
+from timbir_python_amd import synth, xt_hip, xt_hip_f64
 
 
-def test_library_exports_every_declared_symbol():
+@pytest.mark.parametrize("mod,real", [(xt_hip, "float"), (xt_hip_f64, "double")],
+                         ids=["libxt_hip", "libxt_hip_f64"])
+def test_library_exports_every_declared_symbol(mod, real):
     declared = set()
     for h in ("xt_icd.h", "xt_unet.h"):
         header = open(os.path.join(ROOT, "include", h)).read()
         declared |= set(re.findall(r"\b(ICD_BackProject|xt_[a-z0-9_]+)\s*\(", header))
-    assert declared == set(xt_hip.EXPORTS)
-    lib = xt_hip.lib()
+    assert declared == set(mod.EXPORTS)
+    lib = mod.lib()
     for name in declared:
         assert hasattr(lib, name), name
-    assert "gfx950" in xt_hip.version()
+    assert "gfx950" in mod.version() and mod.version().endswith("Real_t=" + real)
+    assert C.sizeof(mod.Sinogram) == C.sizeof(xt_abi_types(real == "double")[0])
+
+
+def xt_abi_types(double):
+    from timbir_python_amd import xt_abi
+    return xt_abi.abi_types(double)
 
 
 def _scalar_fields(st):
@@ -39,15 +47,22 @@ def _scalar_fields(st):
     return out
 
 
+@pytest.mark.parametrize("double", [False, True], ids=["float", "double"])
 @pytest.mark.parametrize("cfg", [TINY, SMALL4D,
                                  dataclasses.replace(TINY, name="tiny_mz2", N_z=2, mult_z=2)],
                          ids=lambda c: c.name)
-def test_engine_setup_matches_oracle(cfg, scan_root, oracle32):
+def test_engine_setup_matches_oracle(cfg, double, scan_root, oracle32, oracle64):
+    """Float build against the float oracle; the double build (Real_t = double,
+    double .bin files) against the oracle's REAL_IS_DOUBLE build: every scalar,
+    array and list equal."""
     path, info = make_scan(cfg, scan_root)
+    mod, oracle = (xt_hip_f64, oracle64) if double else (xt_hip, oracle32)
+    if double:
+        path = as_double(path)
     args = cfg.cli_args()
-    eng = xt_hip.Engine(path, args)
-    S, O, T = oracle32.setup(path, args)
-    oracle32.read_sinogram(path, S, T)
+    eng = mod.Engine(path, args)
+    S, O, T = oracle.setup(path, args)
+    oracle.read_sinogram(path, S, T)
     try:
         for a, b in ((eng.S, S), (eng.O, O), (eng.T, T)):
             fa, fb = _scalar_fields(a), _scalar_fields(b)
@@ -59,23 +74,24 @@ def test_engine_setup_matches_oracle(cfg, scan_root, oracle32):
         assert eng.T.Time_Filter[0] == T.Time_Filter[0]
         n = S.N_p
         for fld in ("ViewPtr", "TimePtr", "cosine", "sine"):
-            ga = np.ctypeslib.as_array((C.c_float * n).from_address(getattr(eng.S, fld)))
-            gb = oracle32.vec_view(getattr(S, fld), n)
+            ga = np.ctypeslib.as_array((mod.REAL * n).from_address(getattr(eng.S, fld)))
+            gb = oracle.vec_view(getattr(S, fld), n)
             assert np.array_equal(ga, gb), fld
-        lists_o, num_o = oracle32.proj_lists(O)
+        lists_o, num_o = oracle.proj_lists(O)
         num_g = np.ctypeslib.as_array((C.c_int32 * O.N_time).from_address(eng.O.ProjNum))
         assert np.array_equal(num_g, num_o)
         top = C.cast(eng.O.ProjIdxPtr, C.POINTER(C.c_void_p))
         for f in range(O.N_time):
             g = np.ctypeslib.as_array((C.c_int32 * int(num_o[f])).from_address(top[f]))
             assert np.array_equal(g, lists_o[f])
-        assert np.array_equal(eng.sino("Projection"), oracle32.sino_view(S.Projection, S))
-        assert np.array_equal(eng.sino("Weight"), oracle32.sino_view(T.Weight, S))
+        assert eng.sino("Projection").dtype == mod.NP_REAL
+        assert np.array_equal(eng.sino("Projection"), oracle.sino_view(S.Projection, S))
+        assert np.array_equal(eng.sino("Weight"), oracle.sino_view(T.Weight, S))
         assert eng.proj_select().all()
-        assert np.array_equal(eng.offset(), oracle32.offset_view(S))
+        assert np.array_equal(eng.offset(), oracle.offset_view(S))
     finally:
         eng.close()
-        oracle32.free(S, O, T)
+        oracle.free(S, O, T)
 
 
 def test_engine_sigma_hu_roundtrip(scan_root):

@@ -46,21 +46,21 @@ std::string join(const char *dir, const char *name) {
   return std::string(dir) + "/" + name;
 }
 
-int read_floats(const char *dir, const char *name, float *dst, size_t n) {
+int read_reals(const char *dir, const char *name, Real_t *dst, size_t n) {
   std::string p = join(dir, name);
   FILE *fp = fopen(p.c_str(), "rb");
   if (!fp) return fail(XT_E_IO, "cannot open %s", p.c_str());
-  size_t r = fread(dst, sizeof(float), n, fp);
+  size_t r = fread(dst, sizeof(Real_t), n, fp);
   fclose(fp);
   if (r != n) return fail(XT_E_IO, "%s holds %zu of %zu values", p.c_str(), r, n);
   return XT_OK;
 }
 
-int write_floats(const char *dir, const char *name, const float *src, size_t n) {
+int write_reals(const char *dir, const char *name, const Real_t *src, size_t n) {
   std::string p = join(dir, name);
   FILE *fp = fopen(p.c_str(), "wb");
   if (!fp) return fail(XT_E_IO, "cannot create %s", p.c_str());
-  size_t w = fwrite(src, sizeof(float), n, fp);
+  size_t w = fwrite(src, sizeof(Real_t), n, fp);
   fclose(fp);
   if (w != n) return fail(XT_E_IO, "short write to %s", p.c_str());
   return XT_OK;
@@ -109,9 +109,9 @@ const Opt kOpts[] = {
     {"x_width", true, O_XW}};
 
 int parse_args(int argc, char **argv, Sinogram *S, ScannedObject *O, TomoInputs *T) {
-  const float slope_HU =
-      (float)((1000 - 0) / (kWaterAtt * kWaterDen - kAirAtt * kAirDen) / kUnit);
-  const float c_HU = (float)(-(double)slope_HU * (kAirAtt * kAirDen * kUnit));
+  const Real_t slope_HU =
+      (Real_t)((1000 - 0) / (kWaterAtt * kWaterDen - kAirAtt * kAirDen) / kUnit);
+  const Real_t c_HU = (Real_t)(-(double)slope_HU * (kAirAtt * kAirDen * kUnit));
   // defaults, F/XT_Init.c:513-531
   T->var_est = 1;
   T->reconstruct = 1;
@@ -136,33 +136,33 @@ int parse_args(int argc, char **argv, Sinogram *S, ScannedObject *O, TomoInputs 
         return fail(XT_E_ARG, "--%s needs a value", o->name);
     }
     switch (o->code) {
-      case O_P: O->MRF_P = (float)atof(v); break;
-      case O_SIGS: O->Sigma_S = (float)(atof(v) - c_HU) / slope_HU; break;
-      case O_CS: O->C_S = (float)atof(v); break;
-      case O_SIGT: O->Sigma_T = (float)(atof(v) - c_HU) / slope_HU; break;
-      case O_CT: O->C_T = (float)atof(v); break;
-      case O_DXY: O->mult_xy = (float)atof(v); break;
-      case O_DZ: O->mult_z = (float)atof(v); break;
-      case O_LR: S->Length_R = (float)atof(v); break;
+      case O_P: O->MRF_P = (Real_t)atof(v); break;
+      case O_SIGS: O->Sigma_S = (Real_t)(atof(v) - c_HU) / slope_HU; break;
+      case O_CS: O->C_S = (Real_t)atof(v); break;
+      case O_SIGT: O->Sigma_T = (Real_t)(atof(v) - c_HU) / slope_HU; break;
+      case O_CT: O->C_T = (Real_t)atof(v); break;
+      case O_DXY: O->mult_xy = (Real_t)atof(v); break;
+      case O_DZ: O->mult_z = (Real_t)atof(v); break;
+      case O_LR: S->Length_R = (Real_t)atof(v); break;
       case O_NTHR: T->num_threads = atoi(v); break;
-      case O_LT: S->Length_T = (float)atof(v); break;
-      case O_VTH: T->StopThreshold = (float)atof(v); break;
+      case O_LT: S->Length_T = (Real_t)atof(v); break;
+      case O_VTH: T->StopThreshold = (Real_t)atof(v); break;
       case O_ITER: T->NumIter = atoi(v); break;
-      case O_ROT: T->RotCenter = (float)atof(v); break;
-      case O_ALPHA: T->alpha = (float)atof(v); break;
+      case O_ROT: T->RotCenter = (Real_t)atof(v); break;
+      case O_ALPHA: T->alpha = (Real_t)atof(v); break;
       case O_TREG: T->time_reg = 1; break;
       case O_SINOBIN: T->sinobin = (uint8_t)atoi(v); break;
       case O_INITICD: T->initICD = (uint8_t)atoi(v); break;
       case O_TIFF: T->Write2Tiff = (uint8_t)atoi(v); break;
       case O_NONOISE: T->No_Projection_Noise = 1; break;
-      case O_RT0: O->Rtime0 = (float)atof(v); break;
-      case O_RTD: O->delta_Rtime = (float)atof(v); break;
+      case O_RT0: O->Rtime0 = (Real_t)atof(v); break;
+      case O_RTD: O->delta_Rtime = (Real_t)atof(v); break;
       case O_RTN: O->N_time = atoi(v); break;
       case O_NP: S->N_p = atoi(v); break;
       case O_NR: S->N_r = atoi(v); break;
       case O_NOREC: T->reconstruct = 0; break;
-      case O_CTH: T->cost_thresh = (float)atof(v); break;
-      case O_RAD: T->radius_obj = (float)atof(v); break;
+      case O_CTH: T->cost_thresh = (Real_t)atof(v); break;
+      case O_RAD: T->radius_obj = (Real_t)atof(v); break;
       case O_SB: S->slice_begin = atoi(v); break;
       case O_SN: S->slice_num = atoi(v); break;
       case O_PHXY: T->phantom_N_xy = atoi(v); break;
@@ -173,12 +173,12 @@ int parse_args(int argc, char **argv, Sinogram *S, ScannedObject *O, TomoInputs 
       case O_NONH: T->no_NHICD = 1; break;
       case O_WPI: T->WritePerIter = 1; break;
       case O_EDGE: T->only_Edge_Updates = 1; break;
-      case O_ZT: T->ErrorSinoThresh = (float)atof(v); break;
-      case O_ZD: T->ErrorSinoDelta = (float)atof(v); break;
+      case O_ZT: T->ErrorSinoThresh = (Real_t)atof(v); break;
+      case O_ZD: T->ErrorSinoDelta = (Real_t)atof(v); break;
       case O_IMU: T->initMagUpMap = 1; break;
       case O_HDF: T->readSino4mHDF = 1; break;
       case O_VAR: T->updateVar = 1; break;
-      case O_VEST: T->var_est = (float)atof(v); break;
+      case O_VEST: T->var_est = (Real_t)atof(v); break;
       case O_RMSE: T->RMSE_converged = 1; break;
       case O_XS: S->x_start = atoi(v); break;
       case O_XW: S->x_width = atoi(v); break;
@@ -216,7 +216,7 @@ int parse_view_info(const char *dir, Sinogram *S) {
   FILE *fp = fopen(p.c_str(), "r");
   if (!fp) return fail(XT_E_IO, "cannot open %s", p.c_str());
   std::string tok;
-  float time = 0;
+  Real_t time = 0;
   int index = 0, c;
   for (;;) {
     tok.clear();
@@ -231,11 +231,11 @@ int parse_view_info(const char *dir, Sinogram *S) {
     }
     if (last == EOF) break;
     if (last == '-') {
-      time = (float)atof(tok.c_str());
+      time = (Real_t)atof(tok.c_str());
     } else if (last == ',') {
       if (!tok.empty()) {
         if (index < S->N_p) {
-          S->ViewPtr[index] = (float)((float)atof(tok.c_str()) * M_PI / 180.0);
+          S->ViewPtr[index] = (Real_t)((Real_t)atof(tok.c_str()) * M_PI / 180.0);
           S->TimePtr[index] = time;
         }
         index++;
@@ -257,7 +257,7 @@ void bin_views(Sinogram *S, ScannedObject *O) {
   O->ProjNum = (int32_t *)calloc(O->N_time, sizeof(int32_t));
   std::vector<int32_t> idx;
   int obj = 0;
-  float t2 = O->Rtime0 + O->delta_Rtime;
+  Real_t t2 = O->Rtime0 + O->delta_Rtime;
   auto flush = [&](int f) {
     O->ProjIdxPtr[f] = (int32_t *)calloc(idx.size() ? idx.size() : 1, sizeof(int32_t));
     if (!idx.empty()) memcpy(O->ProjIdxPtr[f], idx.data(), idx.size() * sizeof(int32_t));
@@ -279,18 +279,18 @@ void bin_views(Sinogram *S, ScannedObject *O) {
 // prior weights proportional to 1/distance, F/XT_Init.c:199-256
 void init_filter(ScannedObject *O, TomoInputs *T) {
   auto dist = [](int i, int j, int k, int l) {
-    return (float)sqrt(pow((float)i - 1.0, 2.0) + pow((float)j - 1.0, 2.0) +
-                       pow((float)k - 1.0, 2.0) + pow((float)l - 1.0, 2.0));
+    return (Real_t)sqrt(pow((Real_t)i - 1.0, 2.0) + pow((Real_t)j - 1.0, 2.0) +
+                       pow((Real_t)k - 1.0, 2.0) + pow((Real_t)l - 1.0, 2.0));
   };
-  float prior_const = O->mult_xy * O->mult_xy * O->mult_xy * O->delta_Rtime;
-  float sum = 0;
-  T->Time_Filter[0] = (float)(1.0 / dist(0, 1, 1, 1));
-  sum += (float)(2.0 * T->Time_Filter[0]);
+  Real_t prior_const = O->mult_xy * O->mult_xy * O->mult_xy * O->delta_Rtime;
+  Real_t sum = 0;
+  T->Time_Filter[0] = (Real_t)(1.0 / dist(0, 1, 1, 1));
+  sum += (Real_t)(2.0 * T->Time_Filter[0]);
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++)
       for (int k = 0; k < 3; k++) {
         if (i != 1 || j != 1 || k != 1) {
-          float t = (float)(1.0 / dist(1, i, j, k));
+          Real_t t = (Real_t)(1.0 / dist(1, i, j, k));
           T->Spatial_Filter[i][j][k] = t;
           sum = sum + t;
         } else {
@@ -318,7 +318,7 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
   if (rc) return rc;
   if (O->N_time < 1) return fail(XT_E_ARG, "Rtime_num must be >= 1");
   if (T->initICD != 0) {
-    rc = read_floats(dir, "variance_estimate_n0.bin", &T->var_est, 1);
+    rc = read_reals(dir, "variance_estimate_n0.bin", &T->var_est, 1);
     if (rc) return rc;
   }
   // initStructures, F/XT_Init.c:293-448 (single rank)
@@ -336,8 +336,8 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
   }
   S->R0 = -T->RotCenter * S->delta_r;
   S->RMax = (S->N_r - T->RotCenter) * S->delta_r;
-  S->T0 = (float)(-S->Length_T / 2.0);
-  S->TMax = (float)(S->Length_T / 2.0);
+  S->T0 = (Real_t)(-S->Length_T / 2.0);
+  S->TMax = (Real_t)(S->Length_T / 2.0);
   O->Length_X = S->Length_R;
   O->Length_Y = S->Length_R;
   O->Length_Z = S->Length_T;
@@ -346,11 +346,11 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
   O->N_z = (int32_t)(S->N_t / O->delta_z);
   O->delta_xy = O->delta_xy * S->delta_r;
   O->delta_z = O->delta_z * S->delta_t;
-  S->Projection = tree<float ***>(3, {S->N_p, S->N_r, S->N_t});
+  S->Projection = tree<Real_t ***>(3, {S->N_p, S->N_r, S->N_t});
   S->ProjSelect = (bool ***)xt_multialloc(sizeof(bool), 3,
                                           std::vector<int64_t>{S->N_p, S->N_r, S->N_t}.data());
-  S->ProjOffset = tree<float **>(2, {S->N_r, S->N_t});
-  T->Weight = tree<float ***>(3, {S->N_p, S->N_r, S->N_t});
+  S->ProjOffset = tree<Real_t **>(2, {S->N_r, S->N_t});
+  T->Weight = tree<Real_t ***>(3, {S->N_p, S->N_r, S->N_t});
   if (!S->Projection || !S->ProjSelect || !S->ProjOffset || !T->Weight)
     return fail(XT_E_ARG, "out of host memory");
   memset(&S->ProjSelect[0][0][0], 1, (size_t)S->N_p * S->N_r * S->N_t);
@@ -360,8 +360,8 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
       // over the pixels it covers (F/XT_Init.c:340-361)
       const int pxy = (T->initICD == 2 || T->initICD == 3) ? 2 : 1, pz = T->initICD == 3 ? 2 : 1;
       const int nr = S->N_r / pxy, nt = S->N_t / pz;
-      std::vector<float> c((size_t)nr * nt);
-      rc = read_floats(dir, "proj_offset_n0.bin", c.data(), c.size());
+      std::vector<Real_t> c((size_t)nr * nt);
+      rc = read_reals(dir, "proj_offset_n0.bin", c.data(), c.size());
       if (rc) return rc;
       for (int i = 0; i < nr; i++)
         for (int j = 0; j < nt; j++)
@@ -369,22 +369,22 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
             for (int q = 0; q < pz; q++)
               S->ProjOffset[i * pxy + p][j * pz + q] = c[(size_t)i * nt + j];
     } else {
-      rc = read_floats(dir, "proj_offset_n0.bin", &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
+      rc = read_reals(dir, "proj_offset_n0.bin", &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
       if (rc) return rc;
     }
   }
   O->x0 = S->R0;
   O->z0 = S->T0;
-  O->y0 = (float)(-O->Length_Y / 2.0);
+  O->y0 = (Real_t)(-O->Length_Y / 2.0);
   O->BeamWidth = S->delta_r;
-  O->Object = (float ****)calloc(O->N_time, sizeof(float ***));
-  for (int i = 0; i < O->N_time; i++) O->Object[i] = tree<float ***>(3, {O->N_z + 2, O->N_y, O->N_x});
-  S->OffsetR = (float)((O->delta_xy / sqrt(2.0) + S->delta_r / 2.0) / 64);
-  S->OffsetT = (float)(((O->delta_z / 2) + S->delta_t / 2) / 64);
-  T->Sigma_S_Q = (float)pow((O->Sigma_S * O->mult_xy), 2.0);
-  T->Sigma_S_Q_P = (float)pow(O->Sigma_S * O->mult_xy, 2.0 - O->MRF_P);
-  T->Sigma_T_Q = (float)pow((O->Sigma_T * O->delta_Rtime), 2.0);
-  T->Sigma_T_Q_P = (float)pow(O->Sigma_T * O->delta_Rtime, 2.0 - O->MRF_P);
+  O->Object = (Real_t ****)calloc(O->N_time, sizeof(Real_t ***));
+  for (int i = 0; i < O->N_time; i++) O->Object[i] = tree<Real_t ***>(3, {O->N_z + 2, O->N_y, O->N_x});
+  S->OffsetR = (Real_t)((O->delta_xy / sqrt(2.0) + S->delta_r / 2.0) / 64);
+  S->OffsetT = (Real_t)(((O->delta_z / 2) + S->delta_t / 2) / 64);
+  T->Sigma_S_Q = (Real_t)pow((O->Sigma_S * O->mult_xy), 2.0);
+  T->Sigma_S_Q_P = (Real_t)pow(O->Sigma_S * O->mult_xy, 2.0 - O->MRF_P);
+  T->Sigma_T_Q = (Real_t)pow((O->Sigma_T * O->delta_Rtime), 2.0);
+  T->Sigma_T_Q_P = (Real_t)pow(O->Sigma_T * O->delta_Rtime, 2.0 - O->MRF_P);
   T->num_z_blocks = (int32_t)(2 * floor(T->num_threads / O->N_time));
   if (T->num_z_blocks < 2)
     T->num_z_blocks = 2;
@@ -399,16 +399,16 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
   T->NHICDSelectNum = tree<int32_t **>(2, {O->N_time, nb});
   O->NHICD_Iterations = 10;
   init_filter(O, T);
-  S->ViewPtr = (float *)calloc(S->N_p > 0 ? S->N_p : 1, sizeof(float));
-  S->TimePtr = (float *)calloc(S->N_p > 0 ? S->N_p : 1, sizeof(float));
+  S->ViewPtr = (Real_t *)calloc(S->N_p > 0 ? S->N_p : 1, sizeof(Real_t));
+  S->TimePtr = (Real_t *)calloc(S->N_p > 0 ? S->N_p : 1, sizeof(Real_t));
   rc = parse_view_info(dir, S);
   if (rc) return rc;
   bin_views(S, O);
-  S->cosine = (float *)calloc(S->N_p, sizeof(float));
-  S->sine = (float *)calloc(S->N_p, sizeof(float));
+  S->cosine = (Real_t *)calloc(S->N_p, sizeof(Real_t));
+  S->sine = (Real_t *)calloc(S->N_p, sizeof(Real_t));
   for (int i = 0; i < S->N_p; i++) {
-    S->cosine[i] = (float)cos(S->ViewPtr[i]);
-    S->sine[i] = (float)sin(S->ViewPtr[i]);
+    S->cosine[i] = (Real_t)cos(S->ViewPtr[i]);
+    S->sine[i] = (Real_t)sin(S->ViewPtr[i]);
   }
   return XT_OK;
 }
@@ -421,23 +421,23 @@ extern "C" int xt_engine_setup(const char *dir, int argc, char **argv, Sinogram 
 static int read_subsampled_counts(const char *dir, Sinogram *S, ScannedObject *O, TomoInputs *T) {
   const int mxy = (int)O->mult_xy, mz = (int)O->mult_z;
   const int NR = S->N_r * mxy, NT = S->N_t * mz;
-  std::vector<float> bright((size_t)NR * NT), weight((size_t)S->N_p * NR * NT);
-  int rc = read_floats(dir, "bright_n0.bin", bright.data(), bright.size());
+  std::vector<Real_t> bright((size_t)NR * NT), weight((size_t)S->N_p * NR * NT);
+  int rc = read_reals(dir, "bright_n0.bin", bright.data(), bright.size());
   if (rc) return rc;
-  rc = read_floats(dir, "weight_n0.bin", weight.data(), weight.size());
+  rc = read_reals(dir, "weight_n0.bin", weight.data(), weight.size());
   if (rc) return rc;
   const double bh = xt_get_option(XT_OPT_BH_QUAD_COEF);
   for (int i = 0; i < S->N_p; i++)
     for (int k = 0; k < S->N_r; k++)
       for (int slice = 0; slice < S->N_t; slice++) {
-        float weight_temp = 0, bright_temp = 0;
+        Real_t weight_temp = 0, bright_temp = 0;
         for (int p = 0; p < mxy; p++)
           for (int q = 0; q < mz; q++) {
             weight_temp += weight[((size_t)i * NR + k * mxy + p) * NT + slice * mz + q];
             bright_temp += bright[(size_t)(k * mxy + p) * NT + slice * mz + q];
           }
-        const float temp = (float)log((double)(bright_temp / weight_temp));
-        S->Projection[i][k][slice] = (float)(bh * temp * temp + temp);
+        const Real_t temp = (Real_t)log((double)(bright_temp / weight_temp));
+        S->Projection[i][k][slice] = (Real_t)(bh * temp * temp + temp);
         T->Weight[i][k][slice] = weight_temp;
       }
   return XT_OK;
@@ -451,11 +451,11 @@ extern "C" int xt_engine_read_sinogram(const char *dir, Sinogram *S, ScannedObje
   if (T->sinobin == 3) {
     rc = read_subsampled_counts(dir, S, O, T);
   } else {
-    rc = read_floats(dir, "projection_n0.bin", &S->Projection[0][0][0], n);
-    if (rc == XT_OK) rc = read_floats(dir, "weight_n0.bin", &T->Weight[0][0][0], n);
+    rc = read_reals(dir, "projection_n0.bin", &S->Projection[0][0][0], n);
+    if (rc == XT_OK) rc = read_reals(dir, "weight_n0.bin", &T->Weight[0][0][0], n);
   }
   if (rc) return rc;
-  float *w = &T->Weight[0][0][0];
+  Real_t *w = &T->Weight[0][0][0];
   for (size_t i = 0; i < n; i++) w[i] /= T->var_est;  // F/XT_Engine.c:120-123
   return XT_OK;
 }
@@ -465,11 +465,11 @@ extern "C" int xt_engine_write_outputs(const char *dir, Sinogram *S, ScannedObje
   char name[128];
   for (int i = 0; i < O->N_time; i++) {
     snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
-    int rc = write_floats(dir, name, &O->Object[i][1][0][0], (size_t)O->N_z * O->N_y * O->N_x);
+    int rc = write_reals(dir, name, &O->Object[i][1][0][0], (size_t)O->N_z * O->N_y * O->N_x);
     if (rc) return rc;
   }
   snprintf(name, sizeof name, "proj_offset_n%d.bin", T->node_rank);
-  return write_floats(dir, name, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
+  return write_reals(dir, name, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t);
 }
 
 extern "C" void xt_engine_free(Sinogram *S, ScannedObject *O, TomoInputs *T) {

@@ -24,8 +24,14 @@
 #include "xt_device.h"
 #include "xt_unet_internal.h"
 
+// Real_t is float (the reference's production build; libxt_hip.so) or, with
+// XT_REAL_IS_DOUBLE, double (its REAL_IS_DOUBLE build; libxt_hip_f64.so).  Only the
+// host side sees it: the device keeps float arrays, and the double build keeps the
+// float build's system matrix (detector table and A columns from float geometry).
 #ifdef XT_REAL_IS_DOUBLE
-#error "libxt_hip is built with Real_t = float (the reference's production build)"
+#define XT_REAL_NAME "double"
+#else
+#define XT_REAL_NAME "float"
 #endif
 
 // ---------------------------------------------------------------------------
@@ -69,7 +75,7 @@ static uint64_t g_opt_gen = 1;
 std::string g_xt_run_dir;  // directory of ICD_BackProject's own files ("" = cwd)
 
 extern "C" const char *xt_last_error(void) { return g_err; }
-extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=float"; }
+extern "C" const char *xt_version(void) { return "libxt_hip gfx950 Real_t=" XT_REAL_NAME; }
 
 extern "C" int xt_set_option(int option, double value) {
   if (option < 1 || option > 19) return set_err(XT_E_ARG, "unknown option %d", option);
@@ -198,7 +204,8 @@ static int make_geom(const Sinogram *S, const ScannedObject *O, const TomoInputs
     g->Sigma_S_Q_P = T->Sigma_S_Q_P;
     g->Sigma_T_Q = T->Sigma_T_Q;
     g->Sigma_T_Q_P = T->Sigma_T_Q_P;
-    memcpy(g->filt, &T->Spatial_Filter[0][0][0], sizeof g->filt);
+    const Real_t *sf = &T->Spatial_Filter[0][0][0];
+    for (int k = 0; k < 27; k++) g->filt[k] = (float)sf[k];
     g->tfilt = T->Time_Filter[0];
     g->alpha = T->alpha;
     g->T = T->ErrorSinoThresh;
@@ -227,6 +234,45 @@ static int dev_alloc(T **p, size_t n) {
   hipError_t e = hipMalloc((void **)p, n * sizeof(T));
   if (e != hipSuccess)
     return set_err(XT_E_HIP, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return XT_OK;
+}
+
+// Host arrays hold Real_t, device arrays float: n values between them, a plain
+// copy when Real_t is float, converted through a host buffer when it is double.
+static hipError_t real_to_dev(float *dst, const Real_t *src, size_t n) {
+  if (sizeof(Real_t) == sizeof(float))
+    return hipMemcpy(dst, src, n * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<float> t(src, src + n);
+  return hipMemcpy(dst, t.data(), n * sizeof(float), hipMemcpyHostToDevice);
+}
+
+static hipError_t dev_to_real(Real_t *dst, const float *src, size_t n) {
+  if (sizeof(Real_t) == sizeof(float))
+    return hipMemcpy(dst, src, n * sizeof(float), hipMemcpyDeviceToHost);
+  std::vector<float> t(n);
+  hipError_t e = hipMemcpy(t.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) std::copy(t.begin(), t.end(), dst);
+  return e;
+}
+
+// The views' cosines, sines (either may be null) and angles on the device.  The
+// float build takes the host's (calculateSinCos in Real_t = float, F/XT_Init.c:
+// 259-272); the double build uses the float build's values -- the angle rounded to
+// float, its cosine in double rounded to float -- so both keep one system matrix.
+static int upload_view_tables(float *cosv, float *sinv, float *viewang, const Sinogram *S) {
+  const int n = S->N_p;
+  std::vector<float> c(n), sn(n), a(n);
+  for (int i = 0; i < n; i++) {
+    a[i] = (float)S->ViewPtr[i];
+    c[i] = sizeof(Real_t) == sizeof(float) ? (float)S->cosine[i] : (float)cos((double)a[i]);
+    sn[i] = sizeof(Real_t) == sizeof(float) ? (float)S->sine[i] : (float)sin((double)a[i]);
+  }
+  hipError_t e = hipSuccess;
+  if (cosv) e = hipMemcpy(cosv, c.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess && sinv) e = hipMemcpy(sinv, sn.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess && viewang)
+    e = hipMemcpy(viewang, a.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_err(XT_E_HIP, "view tables: %s", hipGetErrorString(e));
   return XT_OK;
 }
 
@@ -703,9 +749,7 @@ static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject 
   XT_HIP(hipMemcpy(s->views, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->proj_num, num.data(), num.size() * sizeof(int), hipMemcpyHostToDevice));
   XT_HIP(hipMemcpy(s->view_frame, vf.data(), vf.size() * sizeof(int), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->cosv, S->cosine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->sinv, S->sine, g.N_p * sizeof(float), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, g.N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_TRY(upload_view_tables(s->cosv, s->sinv, s->viewang, S));
   // backprojection weights: each view's share of [0, pi) among its frame's view
   // angles taken modulo pi (half the gap to each neighbour, circularly)
   std::vector<float> wv(g.N_p, 0.f);
@@ -732,12 +776,12 @@ static int upload_views(xt_icd_state *s, const Sinogram *S, const ScannedObject 
 
 // A host [N_p][N_r][N_t] array into dst in the HBM layout (through e when the
 // layout is split by parity; e is set afterwards by the resync).
-static int upload_sino(xt_icd_state *s, float *dst, const float *host) {
+static int upload_sino(xt_icd_state *s, float *dst, const Real_t *host) {
   if (!s->g.pm) {
-    XT_HIP(hipMemcpy(dst, host, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+    XT_HIP(real_to_dev(dst, host, s->nsino));
     return XT_OK;
   }
-  XT_HIP(hipMemcpy(s->e, host, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->e, host, s->nsino));
   hipLaunchKernelGGL(k_sino_layout, dim3(grid_1d(s->nsino)), dim3(256), 0, s->st, s->g, s->e, dst,
                      s->nsino, 1);
   XT_HIP(hipGetLastError());
@@ -751,8 +795,7 @@ static int upload_object(xt_icd_state *s, const ScannedObject *O, float *x_dev) 
   float *tmp = nullptr;
   XT_TRY(dev_alloc(&tmp, per * g.N_time));
   for (int f = 0; f < g.N_time; f++) {
-    hipError_t e = hipMemcpy(tmp + per * f, &O->Object[f][1][0][0], per * sizeof(float),
-                             hipMemcpyHostToDevice);
+    hipError_t e = real_to_dev(tmp + per * f, &O->Object[f][1][0][0], per);
     if (e != hipSuccess) {
       (void)hipFree(tmp);
       return set_err(XT_E_HIP, "object upload: %s", hipGetErrorString(e));
@@ -775,7 +818,7 @@ static int download_object(xt_icd_state *s, ScannedObject *O) {
                      tmp);
   hipError_t e = hipStreamSynchronize(s->st);
   for (int f = 0; f < g.N_time && e == hipSuccess; f++)
-    e = hipMemcpy(&O->Object[f][1][0][0], tmp + per * f, per * sizeof(float), hipMemcpyDeviceToHost);
+    e = dev_to_real(&O->Object[f][1][0][0], tmp + per * f, per);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(XT_E_HIP, "object download: %s", hipGetErrorString(e));
   return XT_OK;
@@ -1098,8 +1141,8 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   std::vector<int> masked;
   for (int j = 0; j < g.N_y; j++)
     for (int k = 0; k < g.N_x; k++) {
-      float xc = (float)((double)O->x0 + ((double)(float)k + 0.5) * (double)O->delta_xy);
-      float yc = (float)((double)O->y0 + ((double)(float)j + 0.5) * (double)O->delta_xy);
+      Real_t xc = (Real_t)((double)O->x0 + ((double)(Real_t)k + 0.5) * (double)O->delta_xy);
+      Real_t yc = (Real_t)((double)O->y0 + ((double)(Real_t)j + 0.5) * (double)O->delta_xy);
       if (xc * xc + yc * yc < T->radius_obj * T->radius_obj) masked.push_back(j * g.N_x + k);
     }
   s->n_masked = (int)masked.size();
@@ -1150,8 +1193,7 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
   if ((rc = upload_sino(s, s->y, &S->Projection[0][0][0])) ||
       (rc = upload_sino(s, s->w, &T->Weight[0][0][0])))
     return fail(rc);
-  if (hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t * sizeof(float),
-                hipMemcpyHostToDevice) != hipSuccess)
+  if (real_to_dev(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t) != hipSuccess)
     return fail(set_err(XT_E_HIP, "sinogram upload failed"));
   if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
       hipMemset(s->magsum, 0, (size_t)g.N_time * s->NYX * sizeof(float)) != hipSuccess ||
@@ -1384,8 +1426,7 @@ extern "C" int xt_icd_upload(xt_icd_state *s, Sinogram *S, ScannedObject *O, Tom
     return set_err(XT_E_ARG, "structs do not match the state's sizes");
   XT_TRY(upload_object(s, O, s->x));
   if (s->edge) XT_TRY(select_edges(s));
-  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)s->g.N_r * s->g.N_t * sizeof(float),
-                   hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->d, &S->ProjOffset[0][0], (size_t)s->g.N_r * s->g.N_t));
   XT_TRY(icd_resync(s));
   XT_HIP(hipStreamSynchronize(s->st));
   return XT_OK;
@@ -1421,8 +1462,7 @@ extern "C" int xt_icd_download(xt_icd_state *s, Sinogram *S, ScannedObject *O, T
   if (!s || !S || !O || !T) return set_err(XT_E_ARG, "null argument");
   XT_HIP(hipStreamSynchronize(s->st));
   XT_TRY(download_object(s, O));
-  XT_HIP(hipMemcpy(&S->ProjOffset[0][0], s->d, (size_t)s->g.N_r * s->g.N_t * sizeof(float),
-                   hipMemcpyDeviceToHost));
+  XT_HIP(dev_to_real(&S->ProjOffset[0][0], s->d, (size_t)s->g.N_r * s->g.N_t));
   if (s->update_var == 1) {
     if (s->g.pm) {
       float *tmp = nullptr;
@@ -1431,11 +1471,11 @@ extern "C" int xt_icd_download(xt_icd_state *s, Sinogram *S, ScannedObject *O, T
                          tmp, s->nsino, 0);
       hipError_t e = hipStreamSynchronize(s->st);
       if (e == hipSuccess)
-        e = hipMemcpy(&T->Weight[0][0][0], tmp, s->nsino * sizeof(float), hipMemcpyDeviceToHost);
+        e = dev_to_real(&T->Weight[0][0][0], tmp, s->nsino);
       (void)hipFree(tmp);
       if (e != hipSuccess) return set_err(XT_E_HIP, "Weight download: %s", hipGetErrorString(e));
     } else {
-      XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+      XT_HIP(dev_to_real(&T->Weight[0][0][0], s->w, s->nsino));
     }
     T->var_est = s->var_est;
   }
@@ -1810,7 +1850,7 @@ static int write_run_bin(const char *name, const void *data, size_t count) {
   std::string p = run_path(name);
   FILE *fp = fopen(p.c_str(), "wb");
   if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
-  size_t w = fwrite(data, sizeof(float), count, fp);
+  size_t w = fwrite(data, sizeof(Real_t), count, fp);
   fclose(fp);
   if (w != count) return set_err(XT_E_IO, "short write to %s", p.c_str());
   return XT_OK;
@@ -1820,7 +1860,7 @@ static int append_run_bin(const char *name, const void *data, size_t count) {
   std::string p = run_path(name);
   FILE *fp = fopen(p.c_str(), "ab");
   if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
-  size_t w = fwrite(data, sizeof(float), count, fp);
+  size_t w = fwrite(data, sizeof(Real_t), count, fp);
   fclose(fp);
   if (w != count) return set_err(XT_E_IO, "short write to %s", p.c_str());
   return XT_OK;
@@ -1830,7 +1870,7 @@ static int read_run_bin(const char *name, void *data, size_t count) {
   std::string p = run_path(name);
   FILE *fp = fopen(p.c_str(), "rb");
   if (!fp) return set_err(XT_E_IO, "cannot open %s", p.c_str());
-  size_t r = fread(data, sizeof(float), count, fp);
+  size_t r = fread(data, sizeof(Real_t), count, fp);
   fclose(fp);
   if (r != count) return set_err(XT_E_IO, "%s holds %zu of %zu values", p.c_str(), r, count);
   return XT_OK;
@@ -1853,7 +1893,7 @@ extern "C" int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, in
   float *buf = nullptr;
   XT_HIP(hipMalloc((void **)&buf, (nc + nx + ny + (init_icd == 3 ? nf : 0)) * sizeof(float)));
   float *dc = buf, *dx = dc + nc, *dy = dx + nx, *dz = dy + ny;
-  hipError_t e = hipMemcpy(dc, coarse, nc * sizeof(float), hipMemcpyHostToDevice);
+  hipError_t e = real_to_dev(dc, coarse, nc);
   auto grid = [](size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)); };
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_upsample_axis, grid(nx), dim3(256), 0, 0, dc, dx, Nzc * Nyc, Nxc, 1, 0);
@@ -1863,7 +1903,7 @@ extern "C" int xt_upsample_object(int32_t init_icd, int32_t Nzc, int32_t Nyc, in
     e = hipGetLastError();
   }
   if (e == hipSuccess)
-    e = hipMemcpy(fine, init_icd == 3 ? dz : dy, nf * sizeof(float), hipMemcpyDeviceToHost);
+    e = dev_to_real(fine, init_icd == 3 ? dz : dy, nf);
   (void)hipFree(buf);
   if (e != hipSuccess) return set_err(XT_E_HIP, "upsampling: %s", hipGetErrorString(e));
   return XT_OK;
@@ -1911,9 +1951,9 @@ extern "C" int xt_counts_to_sinogram(int32_t N_p, int32_t T_in, int32_t R_in, in
                        N_t, rr, rt, bh_quad_coef, dp, dwt);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpy(projection, dp, ns * sizeof(float), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(weight, dwt, ns * sizeof(float), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(bright, db, nb * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = dev_to_real(projection, dp, ns);
+  if (e == hipSuccess) e = dev_to_real(weight, dwt, ns);
+  if (e == hipSuccess) e = dev_to_real(bright, db, nb);
   for (void *p : {(void *)dc, (void *)dw, (void *)dd, (void *)w2, (void *)dp})
     if (p) (void)hipFree(p);
   if (e != hipSuccess) return set_err(XT_E_HIP, "counts to sinogram: %s", hipGetErrorString(e));
@@ -1941,7 +1981,7 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   // magnitude-update map is not read (every line's entry is rewritten in the
   // first iteration before NHICD uses it)
   for (int i = 0; i < O->N_time; i++)
-    memset(&O->Object[i][0][0][0], 0, (size_t)(O->N_z + 2) * O->N_y * O->N_x * sizeof(float));
+    memset(&O->Object[i][0][0][0], 0, (size_t)(O->N_z + 2) * O->N_y * O->N_x * sizeof(Real_t));
   if (T->initICD == 1) {
     for (int i = 0; i < O->N_time; i++) {
       snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
@@ -1952,7 +1992,7 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
       return set_err(XT_E_ARG, "initICD=%d needs even object sizes (%dx%dx%d)", (int)T->initICD,
                      O->N_z, O->N_y, O->N_x);
     const int Nzc = T->initICD == 3 ? O->N_z / 2 : O->N_z;
-    std::vector<float> coarse((size_t)Nzc * (O->N_y / 2) * (O->N_x / 2));
+    std::vector<Real_t> coarse((size_t)Nzc * (O->N_y / 2) * (O->N_x / 2));
     for (int i = 0; i < O->N_time; i++) {
       snprintf(name, sizeof name, "object_n%d_time_%d.bin", T->node_rank, i);
       XT_TRY(read_run_bin(name, coarse.data(), coarse.size()));
@@ -1989,7 +2029,7 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   if (g_opt[XT_OPT_COST] != 0) {
     rc = xt_icd_cost(s, &s->last_cost);
     dbg(T, "ICD_BackProject: cost before ICD %.9g\n", s->last_cost);
-    const float c0 = (float)s->last_cost;
+    const Real_t c0 = (Real_t)s->last_cost;
     if (rc == XT_OK) rc = write_run_bin(costfile, &c0, 1);
   }
   for (int Iter = 1; rc == XT_OK && Iter <= T->NumIter; Iter++) {
@@ -2001,7 +2041,7 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
       if (rc == XT_OK) rc = write_object_offset(S, O, T);
     }
     if (rc == XT_OK && g_opt[XT_OPT_COST] != 0) {
-      const float c = (float)s->last_cost;
+      const Real_t c = (Real_t)s->last_cost;
       rc = append_run_bin(costfile, &c, 1);
     }
     if (rc == XT_OK && s->stopped && Iter > 1) {
@@ -2016,8 +2056,8 @@ extern "C" int ICD_BackProject(Sinogram *S, ScannedObject *O, TomoInputs *T) {
   }
   if (rc == XT_OK) {
     size_t n = (size_t)s->g.N_time * s->g.nb * s->NYX;
-    std::vector<float> mag(n);
-    if (hipMemcpy(mag.data(), s->mag_blk, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<Real_t> mag(n);
+    if (dev_to_real(mag.data(), s->mag_blk, n) != hipSuccess)
       rc = set_err(XT_E_HIP, "magnitude map download");
     snprintf(name, sizeof name, "mag_update_map_n%d.bin", T->node_rank);
     if (rc == XT_OK) rc = write_run_bin(name, mag.data(), n);
@@ -2053,9 +2093,9 @@ static int seam_state(SeamCtx &c, Sinogram *S, ScannedObject *O, TomoInputs *T) 
   return XT_OK;
 }
 
-static int seam_upload_sino(xt_icd_state *s, float **dst, const void *src) {
+static int seam_upload_sino(xt_icd_state *s, float **dst, const Real_t *src) {
   XT_TRY(dev_alloc(dst, s->nsino));
-  XT_HIP(hipMemcpy(*dst, src, s->nsino * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(*dst, src, s->nsino));
   return XT_OK;
 }
 
@@ -2103,11 +2143,10 @@ extern "C" int xt_detector_response(Sinogram *S, ScannedObject *O, TomoInputs *T
   XT_TRY(check_angles(S));
   XT_TRY(dev_alloc(&s->viewang, S->N_p));
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
-  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_TRY(upload_view_tables(nullptr, nullptr, s->viewang, S));
   XT_TRY(launch_detector_response(s, O->BeamWidth));
   XT_HIP(hipStreamSynchronize(s->st));
-  XT_HIP(hipMemcpy(&H_r[0][0], s->Hr, (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
-                   hipMemcpyDeviceToHost));
+  XT_HIP(dev_to_real(&H_r[0][0], s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
   return XT_OK;
 }
 
@@ -2152,10 +2191,8 @@ extern "C" int xt_amatrix_columns(Sinogram *S, ScannedObject *O, Real_t **H_r, i
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
   XT_TRY(dev_alloc(&s->cosv, S->N_p));
   XT_TRY(dev_alloc(&s->sinv, S->N_p));
-  XT_HIP(hipMemcpy(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
-                   hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->cosv, S->cosine, S->N_p * sizeof(float), hipMemcpyHostToDevice));
-  XT_HIP(hipMemcpy(s->sinv, S->sine, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE));
+  XT_TRY(upload_view_tables(s->cosv, s->sinv, nullptr, S));
   int rc = XT_OK;
   if ((rc = dev_alloc(&dr, n)) || (rc = dev_alloc(&dc, n)) || (rc = dev_alloc(&dv, n)) ||
       (rc = dev_alloc(&dn, n)) || (rc = dev_alloc(&di, (size_t)n * max_entries)) ||
@@ -2175,7 +2212,7 @@ extern "C" int xt_amatrix_columns(Sinogram *S, ScannedObject *O, Real_t **H_r, i
     if (e == hipSuccess)
       e = hipMemcpy(index, di, (size_t)n * max_entries * sizeof(int), hipMemcpyDeviceToHost);
     if (e == hipSuccess)
-      e = hipMemcpy(values, dval, (size_t)n * max_entries * sizeof(float), hipMemcpyDeviceToHost);
+      e = dev_to_real(values, dval, (size_t)n * max_entries);
     if (e != hipSuccess) rc = set_err(XT_E_HIP, "amatrix seam: %s", hipGetErrorString(e));
   }
   void *b[] = {dr, dc, dv, dn, di, dval};
@@ -2187,8 +2224,7 @@ extern "C" int xt_amatrix_columns(Sinogram *S, ScannedObject *O, Real_t **H_r, i
 static int seam_geometry_upload(xt_icd_state *s, Sinogram *S, ScannedObject *O, Real_t **H_r) {
   XT_TRY(upload_views(s, S, O));
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
-  XT_HIP(hipMemcpy(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE * sizeof(float),
-                   hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->Hr, &H_r[0][0], (size_t)S->N_p * XT_HR_STRIDE));
   XT_TRY(dev_alloc(&s->x, s->nobj));
   XT_TRY(upload_object(s, O, s->x));
   return XT_OK;
@@ -2225,11 +2261,10 @@ extern "C" int xt_init_error_sino(Sinogram *S, ScannedObject *O, TomoInputs *T, 
   XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
   XT_TRY(dev_alloc(&s->e, s->nsino));
   XT_TRY(dev_alloc(&s->d, (size_t)S->N_r * S->N_t));
-  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t * sizeof(float),
-                   hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->d, &S->ProjOffset[0][0], (size_t)S->N_r * S->N_t));
   XT_TRY(icd_resync(s));
   XT_HIP(hipStreamSynchronize(s->st));
-  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  XT_HIP(dev_to_real(&E[0][0][0], s->e, s->nsino));
   return seam_download_sel(s, S);
 }
 
@@ -2266,7 +2301,7 @@ extern "C" int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
   XT_TRY(seam_upload_sino(s, &s->w, &T->Weight[0][0][0]));
   const int NRT = S->N_r * S->N_t;
   XT_TRY(dev_alloc(&s->d, NRT));
-  XT_HIP(hipMemcpy(s->d, &S->ProjOffset[0][0], NRT * sizeof(float), hipMemcpyHostToDevice));
+  XT_HIP(real_to_dev(s->d, &S->ProjOffset[0][0], NRT));
   int nch = (262144 + NRT - 1) / NRT;
   if (nch > S->N_p) nch = S->N_p;
   s->off_chunk = (S->N_p + nch - 1) / nch;
@@ -2278,8 +2313,8 @@ extern "C" int xt_offset_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
   XT_TRY(dev_alloc(&s->scal, 8));
   XT_TRY(offset_update(s));
   XT_HIP(hipStreamSynchronize(s->st));
-  XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
-  XT_HIP(hipMemcpy(&S->ProjOffset[0][0], s->d, NRT * sizeof(float), hipMemcpyDeviceToHost));
+  XT_HIP(dev_to_real(&E[0][0][0], s->e, s->nsino));
+  XT_HIP(dev_to_real(&S->ProjOffset[0][0], s->d, NRT));
   return seam_download_sel(s, S);
 }
 
@@ -2369,7 +2404,7 @@ extern "C" int xt_variance_update(Sinogram *S, TomoInputs *T, Real_t ***E) {
   XT_TRY(dev_alloc(&s->scal, 8));
   XT_TRY(variance_update(s));
   XT_HIP(hipStreamSynchronize(s->st));
-  XT_HIP(hipMemcpy(&T->Weight[0][0][0], s->w, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+  XT_HIP(dev_to_real(&T->Weight[0][0][0], s->w, s->nsino));
   T->var_est = s->var_est;
   return seam_download_sel(s, S);
 }
@@ -2397,8 +2432,7 @@ extern "C" int xt_vsc_select(ScannedObject *O, TomoInputs *T, Real_t ****MagUpda
       (rc = dev_alloc(&all, NYX)) || (rc = dev_alloc(&cnt, (size_t)M * nch)) ||
       (rc = dev_alloc(&list, M * NYX)) || (rc = dev_alloc(&count, M))) {
   } else {
-    hipError_t e = hipMemcpy(mag, &MagUpdateMap[0][0][0][0], M * NYX * sizeof(float),
-                             hipMemcpyHostToDevice);
+    hipError_t e = real_to_dev(mag, &MagUpdateMap[0][0][0][0], M * NYX);
     if (e == hipSuccess) e = hipMemcpy(nsel, ns.data(), M * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((NYX + 255) / 256), M), dim3(256), 0, 0, NY,
@@ -2477,7 +2511,7 @@ static int update_lines_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_
     XT_HIP(hipMemcpy(s->lines, cols.data(), n_lines * sizeof(int), hipMemcpyHostToDevice));
   XT_TRY(upload_views(s, S, O));
   XT_TRY(dev_alloc(&s->Hr, (size_t)S->N_p * XT_HR_STRIDE));
-  XT_HIP(hipMemcpy(s->viewang, S->ViewPtr, S->N_p * sizeof(float), hipMemcpyHostToDevice));
+  XT_TRY(upload_view_tables(nullptr, nullptr, s->viewang, S));
   XT_TRY(launch_detector_response(s, O->BeamWidth));
   XT_TRY(build_atable(s));
   XT_TRY(dev_alloc(&s->x, s->nobj));
@@ -2530,16 +2564,15 @@ static int update_lines_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, Real_
                        tmp, s->nsino, 0);
     hipError_t e = hipStreamSynchronize(s->st);
     if (e == hipSuccess)
-      e = hipMemcpy(&E[0][0][0], tmp, s->nsino * sizeof(float), hipMemcpyDeviceToHost);
+      e = dev_to_real(&E[0][0][0], tmp, s->nsino);
     (void)hipFree(tmp);
     if (e != hipSuccess) return set_err(XT_E_HIP, "E download: %s", hipGetErrorString(e));
   } else {
-    XT_HIP(hipMemcpy(&E[0][0][0], s->e, s->nsino * sizeof(float), hipMemcpyDeviceToHost));
+    XT_HIP(dev_to_real(&E[0][0][0], s->e, s->nsino));
   }
   XT_TRY(download_object(s, O));
   if (MagUpdate)
-    XT_HIP(hipMemcpy(MagUpdate, s->magsum + (size_t)frame * s->NYX, s->NYX * sizeof(float),
-                     hipMemcpyDeviceToHost));
+    XT_HIP(dev_to_real(MagUpdate, s->magsum + (size_t)frame * s->NYX, s->NYX));
   double st[3];
   XT_HIP(hipMemcpy(st, s->stats, sizeof st, hipMemcpyDeviceToHost));
   if (total_vox_mag) *total_vox_mag = st[0];

@@ -16,7 +16,12 @@ import numpy as np
 from . import xt_abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libxt_hip.so")
+# Real_t of the library this module binds: float (libxt_hip.so, the reference's
+# production build) unless xt_hip_f64 loads this module for the double build
+REAL_IS_DOUBLE = bool(globals().get("_XT_REAL_IS_DOUBLE", False))
+LIB_PATH = os.path.join(HERE, "libxt_hip_f64.so" if REAL_IS_DOUBLE else "libxt_hip.so")
+REAL = C.c_double if REAL_IS_DOUBLE else C.c_float  # host arrays of the structs
+NP_REAL = np.float64 if REAL_IS_DOUBLE else np.float32
 
 (OPT_POSITIVITY, OPT_COST, OPT_RESYNC_EVERY, OPT_SEED, OPT_LINES, OPT_DEVICE, OPT_VERBOSE,
  OPT_OUT_DIR, OPT_ATABLE, OPT_GRAPH, OPT_PACK, OPT_NHICD_LINES, OPT_INIT_FBP,
@@ -56,7 +61,7 @@ class IcdStats(C.Structure):
                 ("line_threads", C.c_int32)]
 
 
-Sinogram, ScannedObject, AMatrixCol, TomoInputs = xt_abi.abi_types(False)
+Sinogram, ScannedObject, AMatrixCol, TomoInputs = xt_abi.abi_types(REAL_IS_DOUBLE)
 
 _lib = None
 
@@ -208,7 +213,7 @@ class Engine:
         """Object[frame] as [N_z+2][N_y][N_x] (halo slices included)."""
         O = self.O
         top = C.cast(O.Object, C.POINTER(C.c_void_p))[frame]
-        return self._view(top, 2, C.c_float, (O.N_z + 2) * O.N_y * O.N_x).reshape(
+        return self._view(top, 2, REAL, (O.N_z + 2) * O.N_y * O.N_x).reshape(
             O.N_z + 2, O.N_y, O.N_x)
 
     def volume(self) -> np.ndarray:
@@ -224,7 +229,7 @@ class Engine:
         S, T = self.S, self.T
         n = S.N_p * S.N_r * S.N_t
         ptr = {"Projection": S.Projection, "Weight": T.Weight}[which]
-        return self._view(ptr, 2, C.c_float, n).reshape(S.N_p, S.N_r, S.N_t)
+        return self._view(ptr, 2, REAL, n).reshape(S.N_p, S.N_r, S.N_t)
 
     def proj_select(self) -> np.ndarray:
         S = self.S
@@ -233,7 +238,7 @@ class Engine:
 
     def offset(self) -> np.ndarray:
         S = self.S
-        return self._view(S.ProjOffset, 1, C.c_float, S.N_r * S.N_t).reshape(S.N_r, S.N_t)
+        return self._view(S.ProjOffset, 1, REAL, S.N_r * S.N_t).reshape(S.N_r, S.N_t)
 
 
 def engine_run(path: str, args) -> None:
@@ -422,7 +427,7 @@ def icd_unet_start(icd: "Icd", net: UNet) -> None:
 
 # -- seams ------------------------------------------------------------------
 def detector_response(eng: Engine) -> np.ndarray:
-    H = np.zeros((eng.S.N_p, 65), np.float32)
+    H = np.zeros((eng.S.N_p, 65), NP_REAL)
     tree = xt_abi.PtrTree(H)
     _check(lib().xt_detector_response(*eng.refs(), tree.ptr))
     return H
@@ -435,8 +440,8 @@ def amatrix_columns(eng: Engine, H, rows, cols, views, max_entries=8):
     n = rows.size
     cnt = np.zeros(n, np.uint8)
     idx = np.zeros((n, max_entries), np.int32)
-    val = np.zeros((n, max_entries), np.float32)
-    ht = xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    val = np.zeros((n, max_entries), NP_REAL)
+    ht = xt_abi.PtrTree(np.ascontiguousarray(H, NP_REAL))
     _check(lib().xt_amatrix_columns(C.byref(eng.S), C.byref(eng.O), ht.ptr, n,
                                     rows.ctypes.data, cols.ctypes.data, views.ctypes.data,
                                     max_entries, cnt.ctypes.data, idx.ctypes.data,
@@ -452,9 +457,9 @@ def counts_to_sinogram(counts, white, dark, N_t: int, N_r: int, bh: float = 0.0)
     white = np.ascontiguousarray(white, np.uint16)
     dark = np.ascontiguousarray(dark, np.uint16)
     n_p, t_in, r_in = counts.shape
-    proj = np.zeros((n_p, N_r, N_t), np.float32)
+    proj = np.zeros((n_p, N_r, N_t), NP_REAL)
     wt = np.zeros_like(proj)
-    bright = np.zeros((N_r, N_t), np.float32)
+    bright = np.zeros((N_r, N_t), NP_REAL)
     _check(lib().xt_counts_to_sinogram(n_p, t_in, r_in, N_t, N_r, counts.ctypes.data,
                                        white.shape[0], white.ctypes.data, dark.shape[0],
                                        dark.ctypes.data, float(bh), proj.ctypes.data,
@@ -465,9 +470,9 @@ def counts_to_sinogram(counts, white, dark, N_t: int, N_r: int, bh: float = 0.0)
 def upsample_object(init_icd: int, coarse) -> np.ndarray:
     """initObject's upsampling of a coarse [Nzc][Nyc][Nxc] object: initICD 2
     doubles y and x, 3 doubles z, y and x (F/XT_ICD_update.c:316-435)."""
-    c = np.ascontiguousarray(coarse, np.float32)
+    c = np.ascontiguousarray(coarse, NP_REAL)
     nz, ny, nx = c.shape
-    out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), np.float32)
+    out = np.zeros((2 * nz if init_icd == 3 else nz, 2 * ny, 2 * nx), NP_REAL)
     _check(lib().xt_upsample_object(init_icd, nz, ny, nx, c.ctypes.data, out.ctypes.data))
     return out
 
@@ -475,7 +480,7 @@ def upsample_object(init_icd: int, coarse) -> np.ndarray:
 def forward_project(eng: Engine, H) -> np.ndarray:
     S = eng.S
     out = np.zeros((S.N_p, S.N_r, S.N_t), np.float64)
-    ht = xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    ht = xt_abi.PtrTree(np.ascontiguousarray(H, NP_REAL))
     _check(lib().xt_forward_project(C.byref(eng.S), C.byref(eng.O), ht.ptr, out.ctypes.data))
     return out
 
@@ -493,14 +498,14 @@ def projector(path: str, args):
 
 def init_error_sino(eng: Engine, H) -> np.ndarray:
     S = eng.S
-    E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
-    et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(np.ascontiguousarray(H, np.float32))
+    E = np.zeros((S.N_p, S.N_r, S.N_t), NP_REAL)
+    et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(np.ascontiguousarray(H, NP_REAL))
     _check(lib().xt_init_error_sino(*eng.refs(), ht.ptr, et.ptr))
     return E
 
 
 def compute_cost(eng: Engine, E) -> float:
-    et = xt_abi.PtrTree(np.ascontiguousarray(E, np.float32))
+    et = xt_abi.PtrTree(np.ascontiguousarray(E, NP_REAL))
     c = C.c_double()
     _check(lib().xt_compute_cost(*eng.refs(), et.ptr, C.byref(c)))
     return c.value
@@ -522,7 +527,7 @@ def vsc_select(eng: Engine, mag: np.ndarray, update_select_num: np.ndarray):
     nb = T.num_z_blocks
     usn = eng._view(T.UpdateSelectNum, 1, C.c_int32, O.N_time * nb).reshape(O.N_time, nb)
     usn[:] = update_select_num
-    mt = xt_abi.PtrTree(np.ascontiguousarray(mag, np.float32))
+    mt = xt_abi.PtrTree(np.ascontiguousarray(mag, NP_REAL))
     _check(lib().xt_vsc_select(C.byref(eng.O), C.byref(eng.T), mt.ptr))
     nxy = O.N_y * O.N_x
     xs = eng._view(T.x_NHICD_select, 2, C.c_int32, O.N_time * nb * nxy).reshape(O.N_time, nb, nxy)
@@ -538,7 +543,7 @@ def update_lines(eng: Engine, E: np.ndarray, frame: int, xs, ys, Iter: int, mask
     xs = np.ascontiguousarray(xs, np.int32)
     ys = np.ascontiguousarray(ys, np.int32)
     O = eng.O
-    mag = np.zeros((O.N_y, O.N_x), np.float32)
+    mag = np.zeros((O.N_y, O.N_x), NP_REAL)
     tot = C.c_double()
     et = xt_abi.PtrTree(E)
     mp = None
@@ -568,7 +573,7 @@ def update_lines_parity(eng: Engine, E: np.ndarray, xs, ys, Iter: int, parity: i
     xs = np.ascontiguousarray(xs, np.int32)
     ys = np.ascontiguousarray(ys, np.int32)
     O = eng.O
-    mag = np.zeros((O.N_y, O.N_x), np.float32)
+    mag = np.zeros((O.N_y, O.N_x), NP_REAL)
     tot = C.c_double()
     et = xt_abi.PtrTree(E)
     _check(lib().xt_update_lines_parity(*eng.refs(), et.ptr, xs.size, xs.ctypes.data,
