@@ -215,6 +215,18 @@ def test_dropin_links_with_reference_host_objects():
                     f"REF={REF_F}"], capture_output=True, text=True, check=True)
 
 
+@pytest.mark.skipif(not os.path.isdir(REF_F), reason="reference sources absent")
+def test_dropin_links_double_host_with_double_library():
+    """A host built with the reference's -DREAL_IS_DOUBLE (double structs) links
+    against libxt_hip_f64.so the same way, with --no-undefined."""
+    r = subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "linkcheck_f64",
+                        f"REF={REF_F}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    so = os.path.join(ROOT, "oracle", "_ref", "libxt_host_link_f64.so")
+    dyn = subprocess.run(["readelf", "-d", so], capture_output=True, text=True).stdout
+    assert "[libxt_hip_f64.so]" in dyn
+
+
 @pytest.mark.parametrize("n_p,n_r,n_t", [(16384, 2048, 16), (8192, 2048, 32), (4096, 8192, 16)])
 def test_oversize_sinogram_refused(n_p, n_r, n_t):
     """The voxel-update kernels address the sinogram through 32-bit byte offsets
@@ -289,21 +301,29 @@ def test_engine_sinobin3_binning_closed_form(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_F), reason="reference sources absent")
+@pytest.mark.parametrize("double", [False, True], ids=["float", "double"])
 @pytest.mark.parametrize("mxy,mz", [(1, 1), (2, 2), (3, 1)])
-def test_engine_sinobin3_matches_reference(mxy, mz, tmp_path, monkeypatch):
+def test_engine_sinobin3_matches_reference(mxy, mz, double, tmp_path, monkeypatch):
     """The engine's counts reader against the reference's own compiled
-    genSinogram_subsampleCounts (oracle/_ref, built with BH_QUAD_COEF 0.05) on the
-    same structs and files: projection and weight bit for bit."""
+    genSinogram_subsampleCounts (oracle/_ref, built with BH_QUAD_COEF 0.05, and
+    with -DREAL_IS_DOUBLE for the double build) on the same structs and files:
+    projection and weight bit for bit."""
     import py_oracle
     r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "gen", f"REF={REF_F}"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
-    ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libxtref_gen_f32.so"))
+    mod = xt_hip_f64 if double else xt_hip
+    ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref",
+                              "libxtref_gen_f64.so" if double else "libxtref_gen_f32.so"))
     ref.genSinogram_subsampleCounts.argtypes = [C.c_void_p] * 3
     d, args, bright, counts = _counts_run(tmp_path, mxy, mz, nr_full=24, nt_full=6, seed=mxy)
-    xt_hip.set_option(xt_hip.OPT_BH_QUAD_COEF, 0.05)
+    if double:
+        for f in ("bright_n0.bin", "weight_n0.bin"):
+            a = np.fromfile(os.path.join(d, f), np.float32)
+            a.astype(np.float64).tofile(os.path.join(d, f))
+    mod.set_option(mod.OPT_BH_QUAD_COEF, 0.05)
     try:
-        with xt_hip.Engine(d, args) as eng:
+        with mod.Engine(d, args) as eng:
             y_ours = eng.sino("Projection").copy()
             w_ours = eng.sino("Weight").copy()
             eng.sino("Projection")[:] = 0
@@ -314,7 +334,7 @@ def test_engine_sinobin3_matches_reference(mxy, mz, tmp_path, monkeypatch):
             y_ref = eng.sino("Projection").copy()
             w_ref = eng.sino("Weight").copy()
     finally:
-        xt_hip.set_option(xt_hip.OPT_BH_QUAD_COEF, 0.0)
+        mod.set_option(mod.OPT_BH_QUAD_COEF, 0.0)
     assert np.abs(y_ref).max() > 0
     np.testing.assert_array_equal(w_ours, w_ref)
     np.testing.assert_array_equal(y_ours, y_ref)
