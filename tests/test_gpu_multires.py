@@ -75,6 +75,26 @@ def test_multires_start_beats_zero_start(scan_root):
     assert err["multires"] < 0.5 * err["zero"]
 
 
+def test_coarse_stages_from_open_sinogram(scan_root):
+    """coarse_stages copying the projection and weights of an open Engine (the
+    final stage's, as the benchmark does) gives the stages the same inputs as
+    reading the run folder's files: with one line in flight (deterministic) the
+    coarse object and offsets are bit-identical."""
+    cfg = dataclasses.replace(SMALL, N_xy=64, seed=12)
+    path, _ = make_scan(cfg, scan_root, "mrcopy")
+    xt_hip.set_option(xt_hip.OPT_LINES, 1)
+    try:
+        print("[gpu] coarse stages from files and from an open engine", flush=True)
+        a = multires.coarse_stages(path, cfg, [3, 2], fbp_first=True)
+        with xt_hip.Engine(path, cfg.cli_args()) as eng:
+            b = multires.coarse_stages(path, cfg, [3, 2], fbp_first=True, sino_from=eng)
+    finally:
+        xt_hip.set_option(xt_hip.OPT_LINES, 0)
+    assert np.abs(a[0]).max() > 0
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
 def test_two_stage_converged_parity(scan_root, oracle32, oracle64a):
     """The launcher's two-stage run (voxels twice as wide, 40 iterations, then
     --initICD 2 at full resolution, 120 iterations) on the GPU and in the C

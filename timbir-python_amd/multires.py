@@ -63,7 +63,7 @@ def stage_configs(cfg: synth.ScanConfig, n_stages: int, first_offsets: bool = Fa
 
 
 def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: bool = False,
-                  quick: bool = True, z_stages: int = 1):
+                  quick: bool = True, z_stages: int = 1, sino_from=None):
     """Runs the stages before the last one (len(iters) of them, iters[k] ICD
     iterations each) on the run folder `path`; returns the last coarse stage's
     object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t].  fbp_first: the
@@ -75,7 +75,10 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     guess gain nothing from either (on the 512^2 x 32 slab: 0.66 s instead of
     0.82 s for 5 + 3 iterations, the same start).  z_stages: the number of z
     levels of the launcher's policy (the returned object then has N_z / 2 slices
-    when the final stage refines z: start_from_coarse with init_icd 3)."""
+    when the final stage refines z: start_from_coarse with init_icd 3).
+    sino_from: an open Engine of the same run folder (the final stage's) whose
+    projection and weights the stages copy instead of reading the files again
+    (every stage reconstructs the same sinogram)."""
     n = len(iters) + 1
     stages = stage_configs(cfg, n, first_offsets=fbp_first, z_stages=z_stages)[:-1]
     inits = init_icd_of(n, z_stages)
@@ -86,7 +89,11 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
             xt_hip.set_option(xt_hip.OPT_AUTOTUNE, 0)
             xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 0)
         for k, (c, n) in enumerate(zip(stages, iters)):
-            with xt_hip.Engine(path, c.cli_args(iterations=n)) as eng:
+            with xt_hip.Engine(path, c.cli_args(iterations=n),
+                               read_sinogram=sino_from is None) as eng:
+                if sino_from is not None:
+                    eng.sino("Projection")[:] = sino_from.sino("Projection")
+                    eng.sino("Weight")[:] = sino_from.sino("Weight")
                 if vol is not None:
                     eng.set_volume(xt_hip.upsample_object(inits[k], vol)[None])
                     eng.offset()[:] = off
