@@ -728,11 +728,13 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
   for (int i = i0; i < i1; i += istep) {
     const int c = list[n_dom ? xt_permute(i, n_dom, key) : i];
     const int yy = c / g.N_x, xx = c - yy * g.N_x;
-    // lane of this thread active: its slice is in the launch's parity and, for
-    // NHICD lines, its own z-block selected the line (F/XT_ICD_update.c:844-881
-    // walk each block's own list)
+    // lane of this thread active: its slice is in the launch's parity, for NHICD
+    // lines its own z-block selected the line (F/XT_ICD_update.c:844-881 walk each
+    // block's own list), and with edge-only updates its voxel is in the mask (an
+    // unmasked voxel is not updated, so its theta is not needed)
     const bool lane_on =
-        zs_ok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblock(zs)) * NYX + c] != 0);
+        zs_ok && (L.mode != 1 || B.nhicd_blk[((size_t)f * g.nb + zblock(zs)) * NYX + c] != 0) &&
+        (!B.vmask || B.vmask[((size_t)f * NYX + c) * g.N_z + zs] != 0);
 
     // ---- 1. stage the A columns (scaled by delta_t) and the neighbourhood
     int cnt_local = 0;
