@@ -1,6 +1,12 @@
 /*
  * xt_icd.h -- C-ABI of libxt_hip.so, the MI355X (gfx950) ICD engine.
  *
+ * Two builds of the same sources: libxt_hip.so with Real_t = float (the
+ * reference's production build) and libxt_hip_f64.so with Real_t = double
+ * (its -DREAL_IS_DOUBLE build; compile the host with XT_REAL_IS_DOUBLE).  Real_t
+ * is the type of every struct field and host array below; device arrays are
+ * float in both.
+ *
  * Plain C: structs from xt_structures.h, raw pointers and sizes; no torch or
  * HIP types.  Every entry point returns 0 on success and a non-zero XT_E*
  * code on failure (the reference exits the process instead; a library must
@@ -38,7 +44,7 @@ enum {
 /* Message for the last non-zero return on this thread ("" if none). */
 const char *xt_last_error(void);
 
-/* Library build identification, e.g. "libxt_hip gfx950 Real_t=float". */
+/* Library build identification: "libxt_hip gfx950 Real_t=float" or "...=double". */
 const char *xt_version(void);
 
 /* ------------------------------------------------------------------------
