@@ -875,14 +875,23 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
         }
       }
     }
-    red1[tid] = th1;
-    red2[tid] = th2;
+    // ---- theta reduction by slice: over the view groups of a wave with lane
+    // shuffles (64 / ZP groups per wave), then over the waves through LDS
+    constexpr int WAVES = NT / 64;
+#pragma unroll
+    for (int m = ZP; m < 64; m <<= 1) {
+      th1 += __shfl_xor(th1, m);
+      th2 += __shfl_xor(th2, m);
+    }
+    if ((tid & 63) < ZP) {
+      red1[(tid >> 6) * ZP + lz] = th1;
+      red2[(tid >> 6) * ZP + lz] = th2;
+    }
     __syncthreads();
-
-    // ---- theta reduction by slice
     if (tid < ZP) {
       float t1 = 0.f, t2 = 0.f;
-      for (int q = 0; q < G; q++) {
+#pragma unroll
+      for (int q = 0; q < WAVES; q++) {
         t1 += red1[q * ZP + tid];
         t2 += red2[q * ZP + tid];
       }
