@@ -231,10 +231,15 @@ def test_variance_update(scan_root, oracle64a):
 
 # 96x96 maps: 9216 lines, three chunks of the line-list compaction (the last partial)
 VSC96 = dataclasses.replace(TINY, name="vsc96", N_xy=96, N_p=48, seed=17)
+# the production slab's maps: 512x512 lines, 64 histogram workgroups and 64
+# compaction chunks per map
+VSC512 = dataclasses.replace(TINY, name="vsc512", N_xy=512, N_p=16, seed=19)
 
 
-@pytest.mark.parametrize("cfg,zero_rows", [(SMALL4D, 3), (VSC96, 3), (VSC96, 90)],
-                         ids=["small4d", "vsc96", "vsc96-threshold-at-zero"])
+@pytest.mark.parametrize("cfg,zero_rows", [(SMALL4D, 3), (VSC96, 3), (VSC96, 90), (VSC512, 3),
+                                           (VSC512, 480)],
+                         ids=["small4d", "vsc96", "vsc96-threshold-at-zero", "vsc512",
+                              "vsc512-threshold-at-zero"])
 def test_vsc_select_bitexact(cfg, zero_rows, scan_root, oracle32):
     """Line sets of the GPU selection against the oracle's (pinned to the
     reference's compiled XT_NHICD.c in test_oracle_pins.py) and, where the
