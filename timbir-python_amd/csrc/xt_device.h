@@ -582,6 +582,22 @@ __device__ __forceinline__ uint32_t xt_hash(uint32_t a) {
   return a;
 }
 
+// Key of the homogeneous line order of one iteration, frame and z-block parity.
+// The reference draws an independent random order for every (frame, z-block)
+// (F/XT_ICD_update.c:599-631); a line visit here updates all slices of one
+// parity at once, so the blocks of a parity share an order, but the two
+// parities take different ones.  With one order for both parities a slice and
+// its z-neighbours (always in the other parity) follow the same in-plane
+// update sequence, their errors stay alike, and the z-prior no longer averages
+// them out: on a 512x512x8 slab with 1024 views the image after three
+// iterations is 21.5% (RMSE / mean attenuation) from the converged one against
+// 16.3% with the reference's per-block orders and 18.5% with per-parity orders
+// (the reference's sequential schedule in every case).
+__device__ __forceinline__ uint32_t xt_order_key(uint32_t seed, int iter, int f, int par) {
+  return xt_hash(seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^
+                                xt_hash((uint32_t)f + 17u + 0x10000u * (uint32_t)(par + 1))));
+}
+
 // A keyed pseudo-random permutation of [0, n): a 4-round Feistel network on
 // the smallest even number of bits covering n, with cycle walking back into
 // range (each walk ends: the cycle through i < n returns below n).
@@ -689,7 +705,7 @@ __device__ __forceinline__ void icd_lines_body(const XtGeom &g, const XtIcdBuffe
     lo = L.sub_iter * per;
     hi = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
     n_dom = n;
-    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+    key = xt_order_key(L.seed, iter, f, par);
   } else if (L.mode == 1) {
     lo = 0;
     hi = B.nhicd_count[f * 2 + par];
@@ -1116,7 +1132,7 @@ __device__ __forceinline__ void icd_groups_body(const XtGeom &g, const XtIcdBuff
     i0 = lo + bx;
     istep = gx;
     n_dom = n;
-    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+    key = xt_order_key(L.seed, iter, f, par);
   } else if (L.mode == 1) {
     const int hi = B.nhicd_count[f * 2 + par];
     list = B.nhicd_list + ((size_t)f * 2 + par) * NYX;
@@ -2174,7 +2190,7 @@ __global__ void __launch_bounds__(256) k_icd_lines_pc(XtGeom g, XtIcdBuffers B, 
     lo = L.sub_iter * per;
     hi = (L.sub_iter == L.K - 1) ? n : (L.sub_iter + 1) * per;
     n_dom = n;
-    key = xt_hash(L.seed ^ xt_hash((uint32_t)iter * 0x85EBCA6Bu ^ xt_hash((uint32_t)f + 17u)));
+    key = xt_order_key(L.seed, iter, f, par);
   } else if (L.mode == 1) {
     hi = B.nhicd_count[f * 2 + par];
     list = B.nhicd_list + ((size_t)f * 2 + par) * NYX;
