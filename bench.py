@@ -190,32 +190,44 @@ def cpu_comparison(cfg, path, xt_hip):
     zero start for the same number of iterations, with the reference's NHICD rule
     (per z-block, XT_OPT_NHICD_LINES 0) and with the default per-parity rule;
     RMSE to the CPU's image after every iteration and the FP64 MAP cost at the
-    end, against the CPU's FP64 cost of its own image."""
+    end, against the CPU's FP64 cost of its own image; where the converged-image
+    sample holds the same voxels, both sides' distance to it after every
+    iteration."""
     if not os.path.exists(CPU_ICD):
         return None
     cpu = np.load(CPU_ICD)
     n = int(cpu["iterations"])
     idx, want, m = cpu["idx"], cpu["values"], float(cpu["mean_att"])
+    conv = np.load(CONVERGED) if os.path.exists(CONVERGED) else None
+    if conv is not None and not np.array_equal(conv["idx"], idx):
+        conv = None
     out = {"iterations": n, "cpu_threads": int(cpu["threads"]),
            "cpu_num_z_blocks": int(cpu["num_z_blocks"]) if "num_z_blocks" in cpu else None,
            "cpu_cost64": float(cpu["cost64"]) if "cost64" in cpu else None, "gpu": {}}
+    if "rmse_to_converged" in cpu:
+        tc = json.loads(str(cpu["rmse_to_converged"]))
+        out["cpu_rmse_to_converged_by_iteration"] = [round(tc[str(k)], 5) for k in range(1, n + 1)
+                                                     if str(k) in tc]
     xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 1)
     for rule in (0, 1):
         xt_hip.set_option(xt_hip.OPT_NHICD_LINES, rule)
         try:
             with xt_hip.Engine(path, cfg.cli_args()) as eng, xt_hip.Icd(eng) as icd:
-                r = []
+                r, rc = [], []
                 for _ in range(n):
                     icd.iterate(1)
                     icd.download()
                     v = eng.volume()[0].reshape(-1)
                     r.append(float(np.sqrt(np.mean((v[idx] - want) ** 2)) / m))
+                    if conv is not None:
+                        rc.append(float(np.sqrt(np.mean((v[idx] - conv["values"]) ** 2)) / m))
                 c = icd.cost()
                 nb = eng.T.num_z_blocks
         finally:
             xt_hip.set_option(xt_hip.OPT_NHICD_LINES, 1)
         key = "nhicd_per_block" if rule == 0 else "nhicd_per_parity"
         out["gpu"][key] = {"rmse_vs_cpu_by_iteration": [round(x, 5) for x in r],
+                           "rmse_to_converged_by_iteration": [round(x, 5) for x in rc],
                            "cost64": c, "num_z_blocks": nb,
                            "cost_rel_to_cpu": ((c - out["cpu_cost64"]) / out["cpu_cost64"]
                                                if out["cpu_cost64"] else None)}

@@ -1,10 +1,12 @@
 """Writes tests/golden/c2_cpu_icd_sample.npz: the CPU ICD (the oracle's C
-restatement of the reference engine, float build, OpenMP) run for 30 iterations
-on the production slab (512x512x32, 1024 views, 768x32 bins; synth.C2), sampled
-at 262144 seeded voxel positions, with the MAP cost of every iteration (the
-float build's own evaluator) and the FP64 MAP cost of the final object and ring
-offsets (the f64a build: the float system matrix in double arithmetic, e = y -
-Ax - d recomputed).  The benchmark compares the GPU's object after the same
+restatement of the reference engine, float build, OpenMP) run for --iters
+iterations (the committed file: 20, 32 threads) on the production slab
+(512x512x32, 1024 views, 768x32 bins; synth.C2), sampled at 262144 seeded voxel
+positions, with the MAP cost of every iteration (the float build's own
+evaluator), the FP64 MAP cost of the final object and ring offsets (the f64a
+build: the float system matrix in double arithmetic, e = y - Ax - d recomputed)
+and the distance of every iteration's object to the converged image of
+c2_converged_sample.npz.  The benchmark compares the GPU's object after the same
 number of iterations, and its FP64 MAP cost, with these.
 
     python tests/golden/make_c2_reference.py RUN_DIR [--threads 32] [--iters 30]
@@ -68,12 +70,22 @@ mean_att = float(ph[ph > 0].mean())
 vol = lambda k: np.fromfile(os.path.join(d, f"object_n0_time_0_iter_{k}.bin"), np.float32)
 ref = vol(a.iters)
 assert ref.size == nvox
-# the CPU run's own distance to its last iteration, as RMSE / mean attenuation
-own = {}
-for k in range(1, a.iters):
+# the CPU run's own distance to its last iteration, as RMSE / mean attenuation, and
+# (where the converged-image fixture samples the same voxels) its distance to the
+# converged image after every iteration
+conv_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2_converged_sample.npz")
+conv = np.load(conv_path) if os.path.exists(conv_path) else None
+if conv is not None and not np.array_equal(conv["idx"], idx):
+    conv = None
+own, to_conv = {}, {}
+for k in range(1, a.iters + 1):
     p = os.path.join(d, f"object_n0_time_0_iter_{k}.bin")
     if os.path.exists(p):
-        own[k] = float(np.sqrt(np.mean((vol(k) - ref) ** 2)) / mean_att)
+        v = vol(k)
+        if k < a.iters:
+            own[k] = float(np.sqrt(np.mean((v - ref) ** 2)) / mean_att)
+        if conv is not None:
+            to_conv[k] = float(np.sqrt(np.mean((v[idx] - conv["values"]) ** 2)) / mean_att)
 
 
 def map_cost64(vol, offset):
@@ -110,5 +122,6 @@ np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2
                     num_z_blocks=np.int32(min(cfg.N_z, max(2, 2 * (a.threads // cfg.N_time)))),
                     layout=np.array("[N_z][N_y][N_x] float32, frame 0"),
                     own_rmse=np.array(json.dumps(own)), cost64=np.float64(cost64),
+                    rmse_to_converged=np.array(json.dumps(to_conv)),
                     costs_f32=costs)
 print("mean attenuation", mean_att, "own RMSE to last:", {k: round(v, 5) for k, v in own.items()})
