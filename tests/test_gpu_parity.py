@@ -403,6 +403,78 @@ def test_update_lines_parity_per_voxel(parity, Iter, cfg, atable, nt, scan_root,
         oracle32.free(S, O, T)
 
 
+# the production slab's image size, 8 slices in 8 one-slice z-blocks
+PROD512 = dataclasses.replace(TINY, name="prod512", N_xy=512, N_z=8, N_p=256, num_threads=4,
+                              seed=27)
+_HAMMING5 = np.array([[0.0064, 0.0432, 0.0800, 0.0432, 0.0064],
+                      [0.0432, 0.2916, 0.5400, 0.2916, 0.0432],
+                      [0.0800, 0.5400, 1.0000, 0.5400, 0.0800],
+                      [0.0432, 0.2916, 0.5400, 0.2916, 0.0432],
+                      [0.0064, 0.0432, 0.0800, 0.0432, 0.0064]], np.float32)
+
+
+@pytest.mark.parametrize("parity", [0, 1])
+def test_update_lines_parity_raster_runs_at_512(parity, scan_root, oracle32):
+    """An NHICD-shaped pass at the production image size: the lines of the top
+    4096 values of the parity's Hamming-filtered |x| map in raster order -- runs of
+    neighbouring lines whose columns share most of their bins, each line reading
+    the previous one's changes -- one after another, the GPU's packed parity pass
+    against the oracle's loop over the parity's blocks (F/XT_ICD_update.c:844-881,
+    F/XT_VoxUpdate.c:85-254).  Sequential dependence amplifies the float
+    summation-order differences along the runs, so the bar is on the RMS
+    difference relative to the pass's own change, with a looser maximum."""
+    from scipy.ndimage import correlate
+    cfg = PROD512
+    _print_case(f"update_lines_parity raster runs parity={parity}", cfg)
+    path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
+    try:
+        nb = T.num_z_blocks
+        assert nb == cfg.N_z
+        # near the data (residuals inside the generalized-Huber threshold): data-driven steps
+        vol = (0.97 * info["phantom"]).astype(np.float32)
+        oracle32.set_object(O, vol)
+        eng.set_volume(vol)
+        H = oracle32.detector_response(S, O, T)
+        E = np.zeros((S.N_p, S.N_r, S.N_t), np.float32)
+        et, ht = xt_abi.PtrTree(E), xt_abi.PtrTree(H)
+        oracle32.lib.oracle_init_error_sino(C.byref(S), C.byref(O), C.byref(T), ht.ptr, et.ptr)
+        Eg = E.copy()
+        filt = correlate(np.abs(vol[0, parity::2]).sum(0), _HAMMING5, mode="constant")
+        th = np.sort(filt.ravel())[-4097]
+        ys, xs = (a.astype(np.int32) for a in np.nonzero(filt > th))
+        n = len(ys)
+        assert 2048 < n <= 4096
+        # most consecutive lines are horizontal neighbours
+        assert np.mean((ys[1:] == ys[:-1]) & (xs[1:] == xs[:-1] + 1)) > 0.8
+        for b in range(parity, nb, 2):
+            mb = np.zeros((O.N_y, O.N_x), np.float32)
+            mt = xt_abi.PtrTree(mb)
+            zc = C.c_int64(0)
+            oracle32.lib.oracle_update_lines(C.byref(S), C.byref(O), C.byref(T), et.ptr, ht.ptr,
+                                             0, b, b, n, xs.ctypes.data, ys.ctypes.data, 2,
+                                             C.byref(zc), mt.ptr, 0)
+        for c0 in range(0, n, 256):
+            xt_hip.update_lines_parity(eng, Eg, xs[c0:c0 + 256], ys[c0:c0 + 256], 2, parity)
+        xo = oracle32.get_object(O)[0]
+        xg = eng.volume()[0]
+        # the updated voxels: the listed lines in the parity's slices
+        sel = (slice(parity, None, 2), ys, xs)
+        d_o = xo[sel] - vol[0][sel]
+        d_g = xg[sel] - xo[sel]
+        change, rms = np.sqrt(np.mean(d_o ** 2)), np.sqrt(np.mean(d_g ** 2))
+        print(f"  {n} lines: change rms {change:.3e} (max {np.abs(d_o).max():.3e}); GPU - oracle "
+              f"rms {rms / change:.3e} of it, max {np.abs(d_g).max() / np.abs(d_o).max():.3e} of the "
+              f"largest change; e {np.abs(Eg - E).max() / np.abs(E).max():.3e}")
+        assert np.array_equal(xg[1 - parity::2], vol[0][1 - parity::2])
+        assert np.abs(d_o).max() > 5e-3 * np.abs(vol).max()
+        assert rms <= 2e-4 * change
+        assert np.abs(d_g).max() <= 2e-3 * np.abs(d_o).max()
+        assert np.abs(Eg - E).max() <= 1e-4 * np.abs(E).max()
+    finally:
+        eng.close()
+        oracle32.free(S, O, T)
+
+
 def _group_lines(O, rng):
     """A line list for the grouped kernel: 2 x 2 and 1 x 2 tiles of adjacent lines
     (columns that share bins), a line listed twice in a row (identical columns),
