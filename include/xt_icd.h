@@ -75,7 +75,8 @@ enum {
                                 NO_COST_CALCULATE (default 0) */
   XT_OPT_RESYNC_EVERY = 3,   /* FP64 recompute of e = y - Ax - d every n iterations; 0 = never
                                 (default 0) */
-  XT_OPT_SEED = 4,           /* seed of the per-tile line order (default 761521) */
+  XT_OPT_SEED = 4,           /* seed of the random line order of every iteration, frame and
+                                z-block parity (default 761521) */
   XT_OPT_LINES = 5,          /* voxel lines of a frame updated at once (workgroups per frame
                                 and launch); 0 = automatic: min(1024, max(16, lines / 64)) */
   XT_OPT_DEVICE = 6,         /* HIP device ordinal (default 0) */
