@@ -403,9 +403,11 @@ def test_update_lines_parity_per_voxel(parity, Iter, cfg, atable, nt, scan_root,
         oracle32.free(S, O, T)
 
 
-# the production slab's image size, 8 slices in 8 one-slice z-blocks
+# the production slab's image size, 8 slices in 8 one-slice z-blocks, and its full
+# z shape (32 slices in 32 one-slice blocks: the benchmark's kernel instantiation)
 PROD512 = dataclasses.replace(TINY, name="prod512", N_xy=512, N_z=8, N_p=256, num_threads=4,
                               seed=27)
+PROD512Z32 = dataclasses.replace(PROD512, name="prod512z32", N_z=32, num_threads=32)
 _HAMMING5 = np.array([[0.0064, 0.0432, 0.0800, 0.0432, 0.0064],
                       [0.0432, 0.2916, 0.5400, 0.2916, 0.0432],
                       [0.0800, 0.5400, 1.0000, 0.5400, 0.0800],
@@ -413,8 +415,9 @@ _HAMMING5 = np.array([[0.0064, 0.0432, 0.0800, 0.0432, 0.0064],
                       [0.0064, 0.0432, 0.0800, 0.0432, 0.0064]], np.float32)
 
 
+@pytest.mark.parametrize("cfg", [PROD512, PROD512Z32], ids=["z8", "z32"])
 @pytest.mark.parametrize("parity", [0, 1])
-def test_update_lines_parity_raster_runs_at_512(parity, scan_root, oracle32):
+def test_update_lines_parity_raster_runs_at_512(parity, cfg, scan_root, oracle32):
     """An NHICD-shaped pass at the production image size: the lines of the top
     4096 values of the parity's Hamming-filtered |x| map in raster order -- runs of
     neighbouring lines whose columns share most of their bins, each line reading
@@ -424,7 +427,6 @@ def test_update_lines_parity_raster_runs_at_512(parity, scan_root, oracle32):
     summation-order differences along the runs, so the bar is on the RMS
     difference relative to the pass's own change, with a looser maximum."""
     from scipy.ndimage import correlate
-    cfg = PROD512
     _print_case(f"update_lines_parity raster runs parity={parity}", cfg)
     path, info, eng, (S, O, T) = _pair(cfg, scan_root, oracle32)
     try:
