@@ -157,7 +157,7 @@ def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
             fbp_first = warm == "multires_fbp"
             vol, off = multires.coarse_stages(
                 path, cfg, MULTIRES_FBP_ITERS if fbp_first else MULTIRES_ITERS,
-                fbp_first=fbp_first, sino_from=eng)
+                fbp_first=fbp_first, icd_from=icd)
             multires.start_from_coarse(icd, vol, off)
         icd.synchronize()
         t_init = time.perf_counter() - t0

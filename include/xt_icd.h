@@ -157,6 +157,13 @@ typedef struct {
 } xt_icd_stats;
 
 int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state);
+/* xt_icd_create with y and w copied on the device from sino_from, a state over the
+ * same sinogram (N_p, N_r, N_t, frames and HBM layout equal, same device), instead of
+ * uploaded from S->Projection and T->Weight, which are not read: for a host that
+ * reconstructs one sinogram at several voxel sizes (the multi-resolution stages), with
+ * no counterpart in the reference.  w is copied as sino_from holds it at the call. */
+int xt_icd_create_from(Sinogram *S, ScannedObject *O, TomoInputs *T,
+                       const xt_icd_state *sino_from, xt_icd_state **state);
 int xt_icd_iterate(xt_icd_state *state, int32_t n_iter, int32_t *stop);
 int xt_icd_set_timing(xt_icd_state *state, int32_t on);
 int xt_icd_get_stats(xt_icd_state *state, xt_icd_stats *stats);

@@ -76,23 +76,42 @@ def test_multires_start_beats_zero_start(scan_root):
 
 
 def test_coarse_stages_from_open_sinogram(scan_root):
-    """coarse_stages copying the projection and weights of an open Engine (the
-    final stage's, as the benchmark does) gives the stages the same inputs as
+    """coarse_stages copying the projection and weights of an open Engine on the
+    host, or y and w of an open state on the device (xt_icd_create_from; the
+    final stage's, as the benchmark does), gives the stages the same inputs as
     reading the run folder's files: with one line in flight (deterministic) the
     coarse object and offsets are bit-identical."""
     cfg = dataclasses.replace(SMALL, N_xy=64, seed=12)
     path, _ = make_scan(cfg, scan_root, "mrcopy")
     xt_hip.set_option(xt_hip.OPT_LINES, 1)
     try:
-        print("[gpu] coarse stages from files and from an open engine", flush=True)
+        print("[gpu] coarse stages from files, from an open engine and from an open state",
+              flush=True)
         a = multires.coarse_stages(path, cfg, [3, 2], fbp_first=True)
         with xt_hip.Engine(path, cfg.cli_args()) as eng:
             b = multires.coarse_stages(path, cfg, [3, 2], fbp_first=True, sino_from=eng)
+            with xt_hip.Icd(eng) as icd:
+                c = multires.coarse_stages(path, cfg, [3, 2], fbp_first=True, icd_from=icd)
     finally:
         xt_hip.set_option(xt_hip.OPT_LINES, 0)
     assert np.abs(a[0]).max() > 0
-    np.testing.assert_array_equal(a[0], b[0])
-    np.testing.assert_array_equal(a[1], b[1])
+    for r in (b, c):
+        np.testing.assert_array_equal(a[0], r[0])
+        np.testing.assert_array_equal(a[1], r[1])
+
+
+def test_create_from_refuses_another_sinogram(scan_root):
+    """xt_icd_create_from takes y and w only from a state of the same sinogram
+    size and layout."""
+    cfg = dataclasses.replace(SMALL, N_xy=64, seed=12)
+    other = dataclasses.replace(cfg, N_p=48, seed=13)
+    p1, _ = make_scan(cfg, scan_root, "cf1")
+    p2, _ = make_scan(other, scan_root, "cf2")
+    print("[gpu] xt_icd_create_from with a source of another size", flush=True)
+    with xt_hip.Engine(p1, cfg.cli_args()) as e1, xt_hip.Icd(e1) as i1, \
+            xt_hip.Engine(p2, other.cli_args()) as e2:
+        with pytest.raises(xt_hip.XtError, match="sinogram source"):
+            xt_hip.Icd(e2, sino_from=i1)
 
 
 def test_two_stage_converged_parity(scan_root, oracle32, oracle64a):

@@ -1104,7 +1104,8 @@ static int select_edges(xt_icd_state *s) {
   return XT_OK;
 }
 
-static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **out) {
+static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **out,
+                       const xt_icd_state *src = nullptr) {
   *out = nullptr;
   xt_icd_state *s = new xt_icd_state();
   auto fail = [&](int rc) {
@@ -1189,10 +1190,26 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
         (rc = dev_alloc(&s->off_wg, (size_t)2 * s->off_nblk)))
       return fail(rc);
   }
-  // the host's y and w, put into the HBM layout through e (set later by the resync)
-  if ((rc = upload_sino(s, s->y, &S->Projection[0][0][0])) ||
-      (rc = upload_sino(s, s->w, &T->Weight[0][0][0])))
-    return fail(rc);
+  if (src) {
+    // y and w of a state over the same sinogram, already in the HBM layout
+    const XtGeom &h = src->g;
+    if (src->dev != s->dev || h.N_p != g.N_p || h.N_r != g.N_r || h.N_t != g.N_t ||
+        h.N_time != g.N_time || h.pm != g.pm || (g.pm && (h.bs != g.bs || h.H != g.H)) ||
+        src->nsino != s->nsino)
+      return fail(set_err(XT_E_ARG, "sinogram source: another device, size or layout"));
+    if (hipStreamSynchronize(src->st) != hipSuccess ||
+        hipMemcpyAsync(s->y, src->y, s->nsino * sizeof(float), hipMemcpyDeviceToDevice, s->st) !=
+            hipSuccess ||
+        hipMemcpyAsync(s->w, src->w, s->nsino * sizeof(float), hipMemcpyDeviceToDevice, s->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess)
+      return fail(set_err(XT_E_HIP, "sinogram copy failed"));
+  } else {
+    // the host's y and w, put into the HBM layout through e (set later by the resync)
+    if ((rc = upload_sino(s, s->y, &S->Projection[0][0][0])) ||
+        (rc = upload_sino(s, s->w, &T->Weight[0][0][0])))
+      return fail(rc);
+  }
   if (real_to_dev(s->d, &S->ProjOffset[0][0], (size_t)g.N_r * g.N_t) != hipSuccess)
     return fail(set_err(XT_E_HIP, "sinogram upload failed"));
   if (hipMemset(s->mag_blk, 0, (size_t)g.N_time * g.nb * s->NYX * sizeof(float)) != hipSuccess ||
@@ -1235,6 +1252,12 @@ static int create_impl(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_stat
 extern "C" int xt_icd_create(Sinogram *S, ScannedObject *O, TomoInputs *T, xt_icd_state **state) {
   if (!S || !O || !T || !state) return set_err(XT_E_ARG, "null argument");
   return create_impl(S, O, T, state);
+}
+
+extern "C" int xt_icd_create_from(Sinogram *S, ScannedObject *O, TomoInputs *T,
+                                  const xt_icd_state *sino_from, xt_icd_state **state) {
+  if (!S || !O || !T || !sino_from || !state) return set_err(XT_E_ARG, "null argument");
+  return create_impl(S, O, T, state, sino_from);
 }
 
 extern "C" int xt_icd_set_timing(xt_icd_state *s, int32_t on) {

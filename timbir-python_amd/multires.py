@@ -63,7 +63,7 @@ def stage_configs(cfg: synth.ScanConfig, n_stages: int, first_offsets: bool = Fa
 
 
 def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: bool = False,
-                  quick: bool = True, z_stages: int = 1, sino_from=None):
+                  quick: bool = True, z_stages: int = 1, sino_from=None, icd_from=None):
     """Runs the stages before the last one (len(iters) of them, iters[k] ICD
     iterations each) on the run folder `path`; returns the last coarse stage's
     object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t].  fbp_first: the
@@ -78,7 +78,9 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     when the final stage refines z: start_from_coarse with init_icd 3).
     sino_from: an open Engine of the same run folder (the final stage's) whose
     projection and weights the stages copy instead of reading the files again
-    (every stage reconstructs the same sinogram)."""
+    (every stage reconstructs the same sinogram).  icd_from: an open Icd of the
+    same sinogram (the final stage's) whose y and w the stages' states copy on the
+    device; the stages then neither read nor copy the sinogram on the host."""
     n = len(iters) + 1
     stages = stage_configs(cfg, n, first_offsets=fbp_first, z_stages=z_stages)[:-1]
     inits = init_icd_of(n, z_stages)
@@ -90,14 +92,14 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
             xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 0)
         for k, (c, n) in enumerate(zip(stages, iters)):
             with xt_hip.Engine(path, c.cli_args(iterations=n),
-                               read_sinogram=sino_from is None) as eng:
-                if sino_from is not None:
+                               read_sinogram=sino_from is None and icd_from is None) as eng:
+                if sino_from is not None and icd_from is None:
                     eng.sino("Projection")[:] = sino_from.sino("Projection")
                     eng.sino("Weight")[:] = sino_from.sino("Weight")
                 if vol is not None:
                     eng.set_volume(xt_hip.upsample_object(inits[k], vol)[None])
                     eng.offset()[:] = off
-                with xt_hip.Icd(eng) as icd:
+                with xt_hip.Icd(eng, sino_from=icd_from) as icd:
                     if vol is None and fbp_first:
                         icd.fbp()
                     icd.iterate(n)

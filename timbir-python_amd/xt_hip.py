@@ -31,7 +31,8 @@ NP_REAL = np.float64 if REAL_IS_DOUBLE else np.float32
 # every symbol include/xt_icd.h and include/xt_unet.h declare
 EXPORTS = [
     "xt_last_error", "xt_version", "ICD_BackProject", "xt_set_option", "xt_get_option",
-    "xt_icd_create", "xt_icd_iterate", "xt_icd_set_timing", "xt_icd_get_stats", "xt_icd_cost",
+    "xt_icd_create", "xt_icd_create_from", "xt_icd_iterate", "xt_icd_set_timing",
+    "xt_icd_get_stats", "xt_icd_cost",
     "xt_icd_resync", "xt_icd_fbp", "xt_icd_download", "xt_icd_synchronize", "xt_icd_destroy",
     "xt_icd_batch_create", "xt_icd_batch_iterate", "xt_icd_batch_destroy",
     "xt_engine_setup", "xt_engine_read_sinogram", "xt_engine_write_outputs", "xt_engine_free",
@@ -81,6 +82,7 @@ def lib():
         L.xt_get_option.argtypes = [C.c_int]
         L.xt_get_option.restype = C.c_double
         L.xt_icd_create.argtypes = [P, P, P, C.POINTER(P)]
+        L.xt_icd_create_from.argtypes = [P, P, P, P, C.POINTER(P)]
         L.xt_icd_iterate.argtypes = [P, C.c_int32, C.POINTER(C.c_int32)]
         L.xt_icd_set_timing.argtypes = [P, C.c_int32]
         L.xt_icd_get_stats.argtypes = [P, C.POINTER(IcdStats)]
@@ -255,10 +257,16 @@ def ICD_BackProject(S, O, T) -> None:
 class Icd:
     """xt_icd_create / iterate / download over an Engine's structs."""
 
-    def __init__(self, eng: Engine):
+    def __init__(self, eng: Engine, sino_from: "Icd" = None):
+        """sino_from: an open state over the same sinogram whose y and w the new
+        state copies on the device (xt_icd_create_from) instead of uploading the
+        Engine's Projection and Weight."""
         self.eng = eng
         self.h = C.c_void_p()
-        _check(lib().xt_icd_create(*eng.refs(), C.byref(self.h)))
+        if sino_from is None:
+            _check(lib().xt_icd_create(*eng.refs(), C.byref(self.h)))
+        else:
+            _check(lib().xt_icd_create_from(*eng.refs(), sino_from.h, C.byref(self.h)))
 
     def iterate(self, n: int = 1) -> bool:
         stop = C.c_int32(0)
