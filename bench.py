@@ -120,7 +120,7 @@ WEIGHTS = os.path.join(PKG, "weights", "unet_warmstart_256x256x16.npz")
 # ICD iterations of the 128^2 and 256^2 stages before the 512^2 one: the reference
 # launcher's policy (zero start, offsets from the second stage) and the FBP-first form
 MULTIRES_ITERS = (10, 5)
-MULTIRES_FBP_ITERS = (5, 3)
+MULTIRES_FBP_ITERS = (8, 2)
 
 
 def gpu_converge(cfg, path, xt_hip, max_iter, warm, tol=0.01):
