@@ -245,6 +245,18 @@ def test_oversize_sinogram_refused(n_p, n_r, n_t):
     assert not state.value
 
 
+def test_create_from_needs_a_source():
+    """xt_icd_create_from without a source state is an argument error, raised
+    before any device call (here: with no GPU)."""
+    S, O, T = xt_hip.Sinogram(), xt_hip.ScannedObject(), xt_hip.TomoInputs()
+    lib = xt_hip.lib()
+    state = C.c_void_p()
+    rc = lib.xt_icd_create_from(C.byref(S), C.byref(O), C.byref(T), None, C.byref(state))
+    assert rc == 1  # XT_E_ARG
+    assert b"null argument" in lib.xt_last_error()
+    assert not state.value
+
+
 def test_line_groups_switched_off():
     """Grouped line updates (XT_OPT_LINE_GROUP 2 and 4) are switched off in this
     build: the option refuses them and stays at 1."""
