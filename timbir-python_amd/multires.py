@@ -69,11 +69,14 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     object [N_z][N_y/2][N_x/2] and ring offsets [N_r][N_t].  fbp_first: the
     coarsest stage starts from its filtered backprojection (with the ring-offset
     guess, and offsets estimated from the first stage on) instead of zero.
-    quick: the coarse states skip the create-time autotuner (the default lines
-    in flight) and the per-iteration FP64 recompute of e (as the reference's
-    float build, which never recomputes it): a few iterations of a starting
-    guess gain nothing from either (on the 512^2 x 32 slab: 0.66 s instead of
-    0.82 s for 5 + 3 iterations, the same start).  z_stages: the number of z
+    quick: the coarse states skip the create-time autotuner and the per-iteration
+    FP64 recompute of e (as the reference's float build, which never recomputes
+    it): a few iterations of a starting guess gain nothing from either (on the
+    512^2 x 32 slab: 0.66 s instead of 0.82 s for 5 + 3 iterations, the same
+    start); and unless XT_OPT_LINES is set they update 3/128 of their lines at
+    once instead of the default 1/64 (384 instead of 256 lines at 128^2: 0.87 ->
+    0.83 s for the slab's 8 + 2 FBP-started stages, the same start within
+    run-to-run spread).  z_stages: the number of z
     levels of the launcher's policy (the returned object then has N_z / 2 slices
     when the final stage refines z: start_from_coarse with init_icd 3).
     sino_from: an open Engine of the same run folder (the final stage's) whose
@@ -85,12 +88,15 @@ def coarse_stages(path: str, cfg: synth.ScanConfig, iters, log=None, fbp_first: 
     stages = stage_configs(cfg, n, first_offsets=fbp_first, z_stages=z_stages)[:-1]
     inits = init_icd_of(n, z_stages)
     vol = off = None
-    saved = {o: xt_hip.get_option(o) for o in (xt_hip.OPT_AUTOTUNE, xt_hip.OPT_RESYNC_EVERY)}
+    saved = {o: xt_hip.get_option(o)
+             for o in (xt_hip.OPT_AUTOTUNE, xt_hip.OPT_RESYNC_EVERY, xt_hip.OPT_LINES)}
     try:
         if quick:
             xt_hip.set_option(xt_hip.OPT_AUTOTUNE, 0)
             xt_hip.set_option(xt_hip.OPT_RESYNC_EVERY, 0)
         for k, (c, n) in enumerate(zip(stages, iters)):
+            if quick and saved[xt_hip.OPT_LINES] == 0:
+                xt_hip.set_option(xt_hip.OPT_LINES, min(1024, max(16, 3 * c.N_xy * c.N_xy // 128)))
             with xt_hip.Engine(path, c.cli_args(iterations=n),
                                read_sinogram=sino_from is None and icd_from is None) as eng:
                 if sino_from is not None and icd_from is None:
