@@ -412,6 +412,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--starts", default="multires_fbp,unet,fbp,multires,zero",
+                    help="comma-separated time-to-converge starts to run (the default: all; "
+                         "the speedups need fbp, the like-for-like CPU comparison runs with "
+                         "the default set only)")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--workload", choices=("slab", "4d", "batch", "unet", "pc", "edge"),
                     default="slab",
@@ -475,7 +479,10 @@ def main():
         like = None
         if not a.no_converge:
             conv = {}
-            for warm in ("multires_fbp", "unet", "fbp", "multires", "zero"):
+            starts = [w for w in a.starts.split(",") if w]
+            for warm in starts:
+                if warm not in ("multires_fbp", "unet", "fbp", "multires", "zero"):
+                    raise SystemExit(f"bench.py: unknown start {warm!r}")
                 c = gpu_converge(cfg, work, xt_hip, a.converge_iters, warm)
                 if c is None:
                     log(f"{warm} start not measured (no converged reference or weights)")
@@ -483,8 +490,10 @@ def main():
                 conv[warm] = c
                 log(f"{warm} start: within 1% RMSE of the converged image "
                     f"after {c['iters']} iterations, {c['seconds']} s")
+            partial = len(starts) != 5
             conv = conv if "fbp" in conv else None
-            like = cpu_comparison(cfg, work, xt_hip) if a.workload == "slab" else None
+            like = (cpu_comparison(cfg, work, xt_hip) if a.workload == "slab" and not partial
+                    else None)
         # the ceiling of the line kernel's e update: its atomic write pattern alone
         atomic_rate = None
         if a.workload == "slab":
